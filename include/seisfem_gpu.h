@@ -1,0 +1,133 @@
+/*
+ * seisfem_gpu.h -- C ABI of the MI355X time loop for SeisFEM's explicit
+ * elastic M-PML masslump solver.
+ *
+ * The library takes the arrays the reference's elastic_wave() builds during
+ * setup (time_evolution/elastic_wave.c:136-313) -- the seven node-indexed CSR
+ * matrices M, S1..S6, the lumped mass, the Dirichlet flags, the four elastic
+ * constants and the six M-PML profiles -- and replaces the step loop
+ * (elastic_wave.c:446-818) with hand-written HIP kernels on one GPU.
+ *
+ * Layouts are the reference's: 0-based CSR (int row pointers of length
+ * node_num+1, ascending int columns, double values), node vectors double[N],
+ * Dirichlet flags int[N] (1 = clamp), station nodes int[] (0-based).
+ * All pointers passed to the sfg_* calls are HOST pointers; device memory is
+ * owned by the library.  Functions return SFG_OK (0) or a negative SFG_E_*
+ * code; sfg_last_error() gives a message.  No function calls exit().
+ */
+#ifndef SEISFEM_GPU_H
+#define SEISFEM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SFG_OK = 0,
+    SFG_E_ARG = -1,      /* invalid argument / inconsistent sizes */
+    SFG_E_HIP = -2,      /* HIP runtime error (message in sfg_last_error) */
+    SFG_E_NOMEM = -4,
+    SFG_E_ENERGY = -6,   /* energy guard fired: sum(U^2) or sum(W^2) > 1e7, or NaN
+                            (elastic_wave.c:779-785 exits with status 1 instead) */
+    SFG_E_ZERO_MASS = -7 /* a zero lumped-mass entry was met (elastic_wave.c:726-727) */
+};
+
+/* Arithmetic of a run.  FP64: bit-identical to the reference's FP64 loop.
+ * FP32: single precision state and arithmetic.  FP32_KAHAN: FP32 with a
+ * compensated displacement update. */
+enum { SFG_FP64 = 0, SFG_FP32 = 1, SFG_FP32_KAHAN = 2 };
+
+/* Last error message of the calling thread ("" if none). */
+const char *sfg_last_error(void);
+/* Selects the HIP device used by later calls of this thread (default 0). */
+int sfg_set_device(int device);
+
+/* ---- drop-in replacements of the reference's hot-path entry points ------- */
+
+/* y = A x on the GPU, row sums sequential in column order: bit-identical to
+ * the reference.  Replaces sparse_matrix/csr_matvec.c:1.  Host pointers;
+ * errors are reported through sfg_last_error() (the reference returns void). */
+void csr_matvec(int Bp_size, int *Bp, int *Bj, double *Bx, double *x, double *y);
+
+/* M-PML damping profiles, computed in FP64 on the GPU with the reference's
+ * region tests and operand order.  Replaces pml/abc_mpml.c:1-4 (same
+ * arguments).  Nodes not referenced by any element get 0 (the reference
+ * leaves them unset). */
+void abc_mpml(int node_num, int element_num, int element_order, int *element_node,
+              double **node_xy, int mpml_nx, int mpml_ny, double h, double xmin, double xmax,
+              double ymin, double ymax, double vp_max, int use_mpml_xmin, int use_mpml_xmax,
+              int use_mpml_ymin, int use_mpml_ymax, double *mpml_dx, double *mpml_dy,
+              double *mpml_dxx, double *mpml_dyy, double *mpml_dxx_pyx, double *mpml_dyy_pxy);
+
+/* Ricker wavelet magnitude*(1-2a)exp(-a), a = (pi f0 (t-t0))^2.
+ * Replaces source_receiver/seismic_source.c:1 (host function, same result). */
+double seismic_source(double f0, double t0, double magnitide, double t);
+
+/* ---- operators, media and runs -------------------------------------------- */
+
+typedef struct sfg_operator sfg_operator; /* M, S1..S6 of one mesh on the device */
+typedef struct sfg_medium sfg_medium;     /* per-node coefficients of one model  */
+typedef struct sfg_run sfg_run;           /* a batch of shots being stepped      */
+
+/* Uploads the seven CSR matrices (index 0 = M, 1..6 = S1..S6, the order of
+ * elastic_wave.c:236-313).  Each keeps its own zero-eliminated pattern on
+ * input; the device stores them on the union pattern (explicit zeros added),
+ * which leaves every row sum bit-identical. */
+int sfg_operator_create(sfg_operator **op, int node_num, const int *const csr_p[7],
+                        const int *const csr_j[7], const double *const csr_x[7]);
+int sfg_operator_nnz(const sfg_operator *op); /* union-pattern nnz */
+void sfg_operator_destroy(sfg_operator *op);
+
+/* Per-node model: c[4] = c11, c13, c33, c44 (elastic_wave.c:206, :585-657),
+ * mass_lump (HRZ, :236-238), dirichlet flags (:136-155), mpml[6] = dx, dy,
+ * dxx, dyy, dxx_pyx, dyy_pxy (:227-228).  The FP32 copy folds the Dirichlet
+ * flag into 1/mass_lump (computed in FP64 on the GPU). */
+int sfg_medium_create(sfg_medium **md, const sfg_operator *op, const double *const c[4],
+                      const double *mass_lump, const int *dirichlet, const double *const mpml[6]);
+void sfg_medium_destroy(sfg_medium *md);
+
+typedef struct {
+    int precision;          /* SFG_FP64 / SFG_FP32 / SFG_FP32_KAHAN */
+    int nshots;             /* shots sharing the operator (right-hand sides) */
+    const int *src_node;    /* [nshots] source node per shot (elastic_wave.c:448) */
+    const int *nsteps;      /* [nshots] executed steps per shot, or NULL = max_steps for all */
+    int max_steps;          /* executed steps (the reference executes step-2) */
+    int rec_num;
+    const int *rec_node;    /* [rec_num] */
+    double dt, f0, t0;
+    double amplitude;       /* 1e10 in the reference (elastic_wave.c:526) */
+    double angle_deg;       /* 90 in the reference (elastic_wave.c:66) */
+    int block;              /* threads per block, 0 = tuned default */
+    int shots_per_lane_group; /* SP in {1,4,16,64}, 0 = chosen from nshots */
+    int guard_every;        /* energy-guard interval in steps, 0 = 50 */
+    int use_graph;          /* 1 = replay captured hipGraphs of guard_every steps */
+} sfg_shot_desc;
+
+/* Creates a run with all state zeroed (elastic_wave.c:466-502). */
+int sfg_run_create(sfg_run **run, const sfg_operator *op, const sfg_medium *md,
+                   const sfg_shot_desc *desc);
+/* Executes up to nsteps further steps (fewer at the end).  Returns the number
+ * of steps executed so far (>= 0) or an error code. */
+int sfg_run_advance(sfg_run *run, int nsteps);
+/* Waits for queued work; returns the executed step count. */
+int sfg_run_sync(sfg_run *run);
+/* Receiver displacement U_now/W_now[rec_node] per executed step
+ * (elastic_wave.c:787-795), host arrays [nshots][max_steps][rec_num]. */
+int sfg_run_seismograms(sfg_run *run, double *u, double *w);
+/* Current U = U1+U2+U3 and W = W1+W2+W3 of one shot (snapshot rows,
+ * elastic_wave.c:797-806), host arrays [node_num]. */
+int sfg_run_snapshot(sfg_run *run, int shot, double *U, double *W);
+/* Energy guard values sum(U^2), sum(W^2) per check: [nchecks][nshots][2]. */
+int sfg_run_energy(sfg_run *run, double *energy, int max_checks);
+/* Average device time per step kernel (ms) measured with HIP events on the
+ * run's stream over the last sfg_run_advance call (0 if unavailable). */
+double sfg_run_last_ms_per_step(const sfg_run *run);
+/* Algorithmic HBM bytes of one step kernel launch (operator + coefficients +
+ * state traffic of the active shots). */
+double sfg_run_step_bytes(const sfg_run *run, int active_shots);
+void sfg_run_destroy(sfg_run *run);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,728 @@
+// C ABI of the GPU time loop: operators, media, runs, and the drop-in
+// replacements of csr_matvec / abc_mpml / seismic_source.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "seisfem_gpu.h"
+#include "seisfem_host.h"
+#include "sfg_internal.h"
+
+using namespace sfg;
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_err;
+static thread_local int g_device = 0;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(SFG_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));                \
+    } while (0)
+
+extern "C" const char *sfg_last_error(void) { return g_err.c_str(); }
+
+extern "C" int sfg_set_device(int device)
+{
+    HIPCHK(hipSetDevice(device));
+    g_device = device;
+    return SFG_OK;
+}
+
+template <class T>
+static int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(SFG_E_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    return SFG_OK;
+}
+
+template <class T>
+static void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// drop-ins
+
+extern "C" double seismic_source(double f0, double t0, double magnitide, double t)
+{
+    // Ricker wavelet (source_receiver/seismic_source.c:37-39); same operand order
+    const double pi = 3.141592653589793238462643;
+    const double a = (pi * f0 * (t - t0)) * (pi * f0 * (t - t0));
+    return magnitide * (1 - 2 * a) * exp(-a);
+}
+
+namespace {
+// Scratch device buffers for the host-pointer drop-ins, grown on demand.
+struct DropinScratch {
+    std::mutex mu;
+    void *buf = nullptr;
+    size_t cap = 0;
+    int get(size_t bytes, void **out)
+    {
+        if (bytes > cap) {
+            if (buf) (void)hipFree(buf);
+            buf = nullptr;
+            cap = 0;
+            hipError_t e = hipMalloc(&buf, bytes);
+            if (e != hipSuccess) return fail(SFG_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+            cap = bytes;
+        }
+        *out = buf;
+        return SFG_OK;
+    }
+};
+DropinScratch g_scratch;
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+} // namespace
+
+static int spmv_host(int Bp_size, const int *Bp, const int *Bj, const double *Bx, const double *x,
+                     double *y)
+{
+    const int n = Bp_size - 1;
+    if (n <= 0 || !Bp || !Bj || !Bx || !x || !y) return fail(SFG_E_ARG, "csr_matvec: bad arguments");
+    const int nnz = Bp[n];
+    if (Bp[0] != 0 || nnz < 0) return fail(SFG_E_ARG, "csr_matvec: row pointers must start at 0");
+    for (int i = 0; i < n; ++i)
+        if (Bp[i + 1] < Bp[i]) return fail(SFG_E_ARG, "csr_matvec: row pointers decrease at row %d", i);
+    for (int k = 0; k < nnz; ++k)
+        if (Bj[k] < 0 || Bj[k] >= n) return fail(SFG_E_ARG, "csr_matvec: column %d out of range", Bj[k]);
+    std::lock_guard<std::mutex> lk(g_scratch.mu);
+    const size_t o_rp = 0, o_cj = align256(o_rp + sizeof(int) * (n + 1));
+    const size_t o_a = align256(o_cj + sizeof(int) * std::max(nnz, 1));
+    const size_t o_x = align256(o_a + sizeof(double) * std::max(nnz, 1));
+    const size_t o_y = align256(o_x + sizeof(double) * n);
+    const size_t total = o_y + sizeof(double) * n;
+    void *base;
+    int rc = g_scratch.get(total, &base);
+    if (rc) return rc;
+    char *b = (char *)base;
+    HIPCHK(hipMemcpy(b + o_rp, Bp, sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+        HIPCHK(hipMemcpy(b + o_cj, Bj, sizeof(int) * nnz, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b + o_a, Bx, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(b + o_x, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(launch_spmv64(n, (const int *)(b + o_rp), (const int *)(b + o_cj), (const double *)(b + o_a),
+                         (const double *)(b + o_x), (double *)(b + o_y), nullptr));
+    HIPCHK(hipMemcpy(y, b + o_y, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return SFG_OK;
+}
+
+extern "C" void csr_matvec(int Bp_size, int *Bp, int *Bj, double *Bx, double *x, double *y)
+{
+    g_err.clear();
+    (void)spmv_host(Bp_size, Bp, Bj, Bx, x, y);
+}
+
+static int mpml_host(int node_num, int element_num, int element_order, const int *element_node,
+                     const double *X, const double *Y, int mpml_nx, int mpml_ny, double h,
+                     double xmin, double xmax, double ymin, double ymax, double vp_max, int uxn,
+                     int uxx, int uyn, int uyx, double *const out[6])
+{
+    (void)element_num;
+    (void)element_order;
+    (void)element_node;
+    if (node_num <= 0 || !X || !Y) return fail(SFG_E_ARG, "abc_mpml: bad arguments");
+    for (int k = 0; k < 6; ++k)
+        if (!out[k]) return fail(SFG_E_ARG, "abc_mpml: null output %d", k);
+    // scalars of pml/abc_mpml.c:50-61, evaluated on the host as the reference does
+    MpmlGeo g;
+    g.pxy = 0.15;
+    g.pyx = 0.15;
+    const double Rcoef = 0.0001;
+    g.tx = h * mpml_nx;
+    g.ty = h * mpml_ny;
+    g.x0 = xmin + g.tx;
+    g.x1 = xmax - g.tx;
+    g.y0 = ymin + g.ty;
+    g.y1 = ymax - g.ty;
+    g.d0x = 3.0 * vp_max * log10(1.0 / Rcoef) / (2.0 * g.tx);
+    g.d0y = 3.0 * vp_max * log10(1.0 / Rcoef) / (2.0 * g.ty);
+    g.use_xmin = uxn;
+    g.use_xmax = uxx;
+    g.use_ymin = uyn;
+    g.use_ymax = uyx;
+    std::lock_guard<std::mutex> lk(g_scratch.mu);
+    const size_t n = (size_t)node_num;
+    void *base;
+    int rc = g_scratch.get(sizeof(double) * n * 8, &base);
+    if (rc) return rc;
+    double *dX = (double *)base, *dY = dX + n, *dO = dY + n;
+    HIPCHK(hipMemcpy(dX, X, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dY, Y, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(launch_mpml(node_num, dX, dY, g, dO, nullptr));
+    for (int k = 0; k < 6; ++k)
+        HIPCHK(hipMemcpy(out[k], dO + k * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return SFG_OK;
+}
+
+extern "C" void abc_mpml(int node_num, int element_num, int element_order, int *element_node,
+                         double **node_xy, int mpml_nx, int mpml_ny, double h, double xmin,
+                         double xmax, double ymin, double ymax, double vp_max, int use_mpml_xmin,
+                         int use_mpml_xmax, int use_mpml_ymin, int use_mpml_ymax, double *mpml_dx,
+                         double *mpml_dy, double *mpml_dxx, double *mpml_dyy, double *mpml_dxx_pyx,
+                         double *mpml_dyy_pxy)
+{
+    g_err.clear();
+    double *const out[6] = {mpml_dx, mpml_dy, mpml_dxx, mpml_dyy, mpml_dxx_pyx, mpml_dyy_pxy};
+    if (!node_xy) {
+        fail(SFG_E_ARG, "abc_mpml: node_xy is null");
+        return;
+    }
+    (void)mpml_host(node_num, element_num, element_order, element_node, node_xy[0], node_xy[1],
+                    mpml_nx, mpml_ny, h, xmin, xmax, ymin, ymax, vp_max, use_mpml_xmin,
+                    use_mpml_xmax, use_mpml_ymin, use_mpml_ymax, out);
+}
+
+// ---------------------------------------------------------------------------
+// operators
+
+struct sfg_operator {
+    int n = 0, nnz = 0, device = 0;
+    int *rp = nullptr, *ci = nullptr;
+    double *v64 = nullptr; // [7][nnz]
+    float *v32 = nullptr;  // [7][nnz]
+};
+
+static int check_csr(int n, const int *p, const int *j, int m)
+{
+    if (!p || !j) return fail(SFG_E_ARG, "matrix %d: null CSR array", m);
+    if (p[0] != 0) return fail(SFG_E_ARG, "matrix %d: row pointers must start at 0", m);
+    for (int i = 0; i < n; ++i) {
+        if (p[i + 1] < p[i]) return fail(SFG_E_ARG, "matrix %d: row pointers decrease at row %d", m, i);
+        for (int k = p[i]; k < p[i + 1]; ++k) {
+            if (j[k] < 0 || j[k] >= n) return fail(SFG_E_ARG, "matrix %d: column out of range in row %d", m, i);
+            if (k > p[i] && j[k] <= j[k - 1])
+                return fail(SFG_E_ARG, "matrix %d: columns not strictly ascending in row %d", m, i);
+        }
+    }
+    return SFG_OK;
+}
+
+extern "C" int sfg_operator_create(sfg_operator **out, int n, const int *const csr_p[7],
+                                   const int *const csr_j[7], const double *const csr_x[7])
+{
+    g_err.clear();
+    if (!out || n <= 0 || !csr_p || !csr_j || !csr_x) return fail(SFG_E_ARG, "operator: bad arguments");
+    *out = nullptr;
+    long long sum = 0;
+    for (int m = 0; m < 7; ++m) {
+        if (!csr_x[m]) return fail(SFG_E_ARG, "matrix %d: null values", m);
+        int rc = check_csr(n, csr_p[m], csr_j[m], m);
+        if (rc) return rc;
+        sum += csr_p[m][n];
+    }
+    const int cap = (int)std::min<long long>(sum, 0x7fffffffLL);
+    std::vector<int> up(n + 1), uj(cap);
+    std::vector<double> ux((size_t)7 * cap);
+    double *uxp[7];
+    for (int m = 0; m < 7; ++m) uxp[m] = ux.data() + (size_t)m * cap;
+    const int nnz = sfh_csr_union(n, 7, csr_p, csr_j, csr_x, cap, up.data(), uj.data(), uxp);
+    if (nnz < 0) return fail(SFG_E_ARG, "operator: union pattern failed");
+    sfg_operator *op = new sfg_operator;
+    op->n = n;
+    op->nnz = nnz;
+    op->device = g_device;
+    int rc = SFG_OK;
+    if ((rc = dalloc(&op->rp, n + 1)) || (rc = dalloc(&op->ci, nnz)) ||
+        (rc = dalloc(&op->v64, (size_t)7 * nnz)) || (rc = dalloc(&op->v32, (size_t)7 * nnz))) {
+        sfg_operator_destroy(op);
+        return rc;
+    }
+    hipError_t e = hipMemcpy(op->rp, up.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(op->ci, uj.data(), sizeof(int) * nnz, hipMemcpyHostToDevice);
+    for (int m = 0; m < 7 && e == hipSuccess; ++m)
+        e = hipMemcpy(op->v64 + (size_t)m * nnz, uxp[m], sizeof(double) * nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_to_f32(op->v64, op->v32, (size_t)7 * nnz, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        sfg_operator_destroy(op);
+        return fail(SFG_E_HIP, "operator upload: %s", hipGetErrorString(e));
+    }
+    *out = op;
+    return SFG_OK;
+}
+
+extern "C" int sfg_operator_nnz(const sfg_operator *op) { return op ? op->nnz : 0; }
+
+extern "C" void sfg_operator_destroy(sfg_operator *op)
+{
+    if (!op) return;
+    dfree(op->rp);
+    dfree(op->ci);
+    dfree(op->v64);
+    dfree(op->v32);
+    delete op;
+}
+
+// ---------------------------------------------------------------------------
+// media
+
+struct sfg_medium {
+    int n = 0;
+    double *coef64 = nullptr; // [NCOEF][n], MASS = mass_lump
+    float *coef32 = nullptr;  // [NCOEF][n], MASS = 1/mass_lump, 0 where clamped
+    int *flag = nullptr;      // [n]
+    int zero_mass = 0;
+};
+
+extern "C" int sfg_medium_create(sfg_medium **out, const sfg_operator *op, const double *const c[4],
+                                 const double *mass_lump, const int *dirichlet,
+                                 const double *const mpml[6])
+{
+    g_err.clear();
+    if (!out || !op || !c || !mass_lump || !dirichlet || !mpml) return fail(SFG_E_ARG, "medium: bad arguments");
+    *out = nullptr;
+    const int n = op->n;
+    for (int k = 0; k < 4; ++k)
+        if (!c[k]) return fail(SFG_E_ARG, "medium: null c[%d]", k);
+    for (int k = 0; k < 6; ++k)
+        if (!mpml[k]) return fail(SFG_E_ARG, "medium: null mpml[%d]", k);
+    std::vector<double> h((size_t)NCOEF * n);
+    std::vector<float> h32((size_t)NCOEF * n);
+    int zero = 0;
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 4; ++k) h[(size_t)k * n + i] = c[k][i];
+        for (int k = 0; k < 6; ++k) h[(size_t)(DX + k) * n + i] = mpml[k][i];
+        h[(size_t)MASS * n + i] = mass_lump[i];
+        if (mass_lump[i] == 0.0) ++zero;
+    }
+    // FP32 copy: coefficients rounded once; 1/mass_lump formed in FP64 then
+    // rounded, with the Dirichlet clamp folded in as a zero factor.
+    for (size_t q = 0; q < (size_t)MASS * n; ++q) h32[q] = (float)h[q];
+    for (int i = 0; i < n; ++i)
+        h32[(size_t)MASS * n + i] =
+            (dirichlet[i] == 1 || mass_lump[i] == 0.0) ? 0.0f : (float)(1.0 / mass_lump[i]);
+    sfg_medium *md = new sfg_medium;
+    md->n = n;
+    md->zero_mass = zero;
+    int rc = SFG_OK;
+    if ((rc = dalloc(&md->coef64, (size_t)NCOEF * n)) || (rc = dalloc(&md->coef32, (size_t)NCOEF * n)) ||
+        (rc = dalloc(&md->flag, n))) {
+        sfg_medium_destroy(md);
+        return rc;
+    }
+    hipError_t e = hipMemcpy(md->coef64, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(md->coef32, h32.data(), sizeof(float) * h32.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(md->flag, dirichlet, sizeof(int) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        sfg_medium_destroy(md);
+        return fail(SFG_E_HIP, "medium upload: %s", hipGetErrorString(e));
+    }
+    *out = md;
+    return SFG_OK;
+}
+
+extern "C" void sfg_medium_destroy(sfg_medium *md)
+{
+    if (!md) return;
+    dfree(md->coef64);
+    dfree(md->coef32);
+    dfree(md->flag);
+    delete md;
+}
+
+// ---------------------------------------------------------------------------
+// runs
+
+struct sfg_run {
+    const sfg_operator *op = nullptr;
+    const sfg_medium *md = nullptr;
+    int prec = 0, nshots = 0, S = 1, SP = 1, block = 256, nbx = 1, nby = 1;
+    int max_steps = 0, rec_num = 0, guard_every = 50, nslots = 1;
+    bool use_graph = true;
+    std::vector<int> perm;      // internal shot -> user shot
+    std::vector<int> steps_int; // executed steps per internal shot
+    void *state = nullptr, *comp = nullptr;
+    Ctrl *ctrl = nullptr;
+    int *active = nullptr, *src = nullptr, *rec_ptr = nullptr, *rec_idx = nullptr;
+    double *wavelet = nullptr, *seis_u = nullptr, *seis_w = nullptr, *energy = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int executed = 0;
+    double last_ms = 0.0;
+    StepArgs<double> a64;
+    StepArgs<float> a32;
+};
+
+static void run_free(sfg_run *r)
+{
+    if (r->graph) (void)hipGraphExecDestroy(r->graph);
+    if (r->ev0) (void)hipEventDestroy(r->ev0);
+    if (r->ev1) (void)hipEventDestroy(r->ev1);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    dfree(r->state);
+    dfree(r->comp);
+    dfree(r->ctrl);
+    dfree(r->active);
+    dfree(r->src);
+    dfree(r->rec_ptr);
+    dfree(r->rec_idx);
+    dfree(r->wavelet);
+    dfree(r->seis_u);
+    dfree(r->seis_w);
+    dfree(r->energy);
+    delete r;
+}
+
+extern "C" void sfg_run_destroy(sfg_run *r)
+{
+    if (!r) return;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    run_free(r);
+}
+
+static int launch_one_step(sfg_run *r)
+{
+    const void *args = (r->prec == 0) ? (const void *)&r->a64 : (const void *)&r->a32;
+    HIPCHK(launch_step(r->prec, r->SP, false, args, r->nbx, r->nby, r->block, r->stream));
+    return SFG_OK;
+}
+
+static int launch_guard_now(sfg_run *r)
+{
+    const size_t F = (size_t)r->op->n * r->S;
+    HIPCHK(launch_guard(r->prec, r->state, r->op->n, r->S, F, r->ctrl, r->energy, r->nslots,
+                        r->guard_every, r->stream));
+    return SFG_OK;
+}
+
+extern "C" int sfg_run_create(sfg_run **out, const sfg_operator *op, const sfg_medium *md,
+                              const sfg_shot_desc *d)
+{
+    g_err.clear();
+    if (!out || !op || !md || !d) return fail(SFG_E_ARG, "run: bad arguments");
+    *out = nullptr;
+    if (md->n != op->n) return fail(SFG_E_ARG, "run: medium and operator sizes differ");
+    if (d->precision < 0 || d->precision > 2) return fail(SFG_E_ARG, "run: unknown precision %d", d->precision);
+    if (d->nshots <= 0 || !d->src_node) return fail(SFG_E_ARG, "run: need >= 1 shot with a source node");
+    if (d->max_steps <= 0) return fail(SFG_E_ARG, "run: max_steps must be > 0");
+    if (d->rec_num < 0 || (d->rec_num > 0 && !d->rec_node)) return fail(SFG_E_ARG, "run: bad receivers");
+    if (d->precision != 0 && md->zero_mass)
+        return fail(SFG_E_ZERO_MASS, "run: %d zero lumped-mass entries (FP32 paths need mass > 0)", md->zero_mass);
+    const int n = op->n;
+    for (int s = 0; s < d->nshots; ++s)
+        if (d->src_node[s] < -1 || d->src_node[s] >= n) return fail(SFG_E_ARG, "run: source node out of range");
+    for (int q = 0; q < d->rec_num; ++q)
+        if (d->rec_node[q] < 0 || d->rec_node[q] >= n)
+            return fail(SFG_E_ARG, "run: receiver %d node %d out of range (the reference reads U[-1])", q, d->rec_node[q]);
+    std::vector<int> steps(d->nshots);
+    for (int s = 0; s < d->nshots; ++s) {
+        steps[s] = d->nsteps ? d->nsteps[s] : d->max_steps;
+        if (steps[s] < 0 || steps[s] > d->max_steps) return fail(SFG_E_ARG, "run: nsteps[%d] out of range", s);
+    }
+
+    sfg_run *r = new sfg_run;
+    r->op = op;
+    r->md = md;
+    r->prec = d->precision;
+    r->nshots = d->nshots;
+    r->max_steps = d->max_steps;
+    r->rec_num = d->rec_num;
+    r->guard_every = d->guard_every > 0 ? d->guard_every : 50;
+    r->use_graph = d->use_graph != 0;
+    // lane grouping: shots per lane group and the padded shot stride
+    int sp = d->shots_per_lane_group;
+    if (sp == 0) sp = d->nshots <= 1 ? 1 : d->nshots <= 4 ? 4 : d->nshots <= 16 ? 16 : 64;
+    if (sp != 1 && sp != 4 && sp != 16 && sp != 64) {
+        run_free(r);
+        return fail(SFG_E_ARG, "run: shots_per_lane_group must be 1, 4, 16 or 64");
+    }
+    if (sp < 64 && d->nshots > sp) {
+        run_free(r);
+        return fail(SFG_E_ARG, "run: %d shots do not fit a lane group of %d", d->nshots, sp);
+    }
+    r->SP = sp;
+    r->S = sp < 64 ? sp : 64 * ((d->nshots + 63) / 64);
+    r->nby = sp < 64 ? 1 : r->S / 64;
+    if ((size_t)n * r->S >= (size_t)1 << 31) {
+        run_free(r);
+        return fail(SFG_E_ARG, "run: node_num * padded shots must stay below 2^31");
+    }
+    r->block = d->block > 0 ? d->block : (sp == 1 && n < 256 * 1024 ? 64 : 256);
+    if (r->block % 64 != 0 || r->block > 1024) {
+        run_free(r);
+        return fail(SFG_E_ARG, "run: block must be a multiple of 64, <= 1024");
+    }
+    const int rows_per_block = (r->block / 64) * (sp >= 64 ? 1 : 64 / sp);
+    r->nbx = (n + rows_per_block - 1) / rows_per_block;
+
+    // shots sorted by decreasing step count, so the active shots are a prefix
+    r->perm.resize(d->nshots);
+    std::iota(r->perm.begin(), r->perm.end(), 0);
+    std::stable_sort(r->perm.begin(), r->perm.end(), [&](int x, int y) { return steps[x] > steps[y]; });
+    r->steps_int.resize(d->nshots);
+    std::vector<int> src_int(r->S, -1);
+    for (int s = 0; s < d->nshots; ++s) {
+        r->steps_int[s] = steps[r->perm[s]];
+        src_int[s] = d->src_node[r->perm[s]];
+    }
+    std::vector<int> act(d->max_steps);
+    for (int t = 0; t < d->max_steps; ++t) {
+        int c = 0;
+        while (c < d->nshots && r->steps_int[c] > t) ++c;
+        act[t] = c;
+    }
+    // Ricker table: step t of the run is the reference's it = t + 2, time (it+1)*dt
+    std::vector<double> wav(d->max_steps);
+    for (int t = 0; t < d->max_steps; ++t) wav[t] = seismic_source(d->f0, d->t0, d->amplitude, (t + 3) * d->dt);
+    // receivers grouped by node
+    std::vector<int> rptr(n + 1, 0), ridx(std::max(d->rec_num, 1));
+    for (int q = 0; q < d->rec_num; ++q) rptr[d->rec_node[q] + 1]++;
+    for (int i = 0; i < n; ++i) rptr[i + 1] += rptr[i];
+    {
+        std::vector<int> fillp(rptr.begin(), rptr.end() - 1);
+        for (int q = 0; q < d->rec_num; ++q) ridx[fillp[d->rec_node[q]]++] = q;
+    }
+    r->nslots = (d->max_steps + r->guard_every - 1) / r->guard_every + 1;
+
+    const size_t F = (size_t)n * r->S;
+    const size_t w = r->prec == 0 ? 8 : 4;
+    const size_t seis_n = (size_t)r->S * d->max_steps * std::max(d->rec_num, 1);
+    int rc = SFG_OK;
+    hipError_t e = hipSuccess;
+    if ((rc = dalloc((char **)&r->state, 2 * NFIELD * F * w)) ||
+        (r->prec == 2 && (rc = dalloc((char **)&r->comp, 2 * 6 * F * w))) || (rc = dalloc(&r->ctrl, 1)) ||
+        (rc = dalloc(&r->active, d->max_steps)) || (rc = dalloc(&r->src, r->S)) ||
+        (rc = dalloc(&r->rec_ptr, n + 1)) || (rc = dalloc(&r->rec_idx, ridx.size())) ||
+        (rc = dalloc(&r->wavelet, d->max_steps)) || (rc = dalloc(&r->seis_u, seis_n)) ||
+        (rc = dalloc(&r->seis_w, seis_n)) || (rc = dalloc(&r->energy, (size_t)r->nslots * r->S * 2))) {
+        run_free(r);
+        return rc;
+    }
+    e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&r->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&r->ev1);
+    if (e == hipSuccess) e = launch_fill_zero(r->state, 2 * NFIELD * F * w, r->stream);
+    if (e == hipSuccess && r->comp) e = launch_fill_zero(r->comp, 2 * 6 * F * w, r->stream);
+    if (e == hipSuccess) e = launch_fill_zero(r->ctrl, sizeof(Ctrl), r->stream);
+    if (e == hipSuccess) e = launch_fill_zero(r->seis_u, seis_n * 8, r->stream);
+    if (e == hipSuccess) e = launch_fill_zero(r->seis_w, seis_n * 8, r->stream);
+    if (e == hipSuccess) e = launch_fill_zero(r->energy, (size_t)r->nslots * r->S * 16, r->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->active, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->src, src_int.data(), sizeof(int) * r->S, hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->rec_ptr, rptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->rec_idx, ridx.data(), sizeof(int) * ridx.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->wavelet, wav.data(), sizeof(double) * wav.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) {
+        run_free(r);
+        return fail(SFG_E_HIP, "run setup: %s", hipGetErrorString(e));
+    }
+
+    // kernel arguments (identical for every step: the step index lives on the device)
+    const double pi_ref = 3.1415926535898; // elastic_wave.c:67
+    const double su = sin(d->angle_deg * pi_ref / 180.0), sw = cos(d->angle_deg * pi_ref / 180.0);
+    const double alpha = 1.0, delta = 1.5; // elastic_wave.c:97
+    auto fill = [&](auto &a, auto *state, auto *comp, auto *const vals, auto *coef, auto cast) {
+        a.n = n;
+        a.S = r->S;
+        a.rows_per_block = rows_per_block;
+        a.nsteps_max = d->max_steps;
+        a.rp = op->rp;
+        a.ci = op->ci;
+        for (int m = 0; m < 7; ++m) a.A[m] = vals + (size_t)m * op->nnz;
+        a.coef = coef;
+        a.flag = md->flag;
+        a.state = state;
+        a.comp = comp;
+        a.fstride = F;
+        a.active = r->active;
+        a.src = r->src;
+        a.wavelet = r->wavelet;
+        a.src_u = su;
+        a.src_w = sw;
+        a.dt = cast(d->dt);
+        a.ca = cast(0.5 - alpha);
+        a.alpha = cast(alpha);
+        a.cd = cast(1 - delta);
+        a.delta = cast(delta);
+        a.rec_ptr = r->rec_ptr;
+        a.rec_idx = r->rec_idx;
+        a.rec_num = d->rec_num;
+        a.seis_u = r->seis_u;
+        a.seis_w = r->seis_w;
+        a.ctrl = r->ctrl;
+    };
+    if (r->prec == 0)
+        fill(r->a64, (double *)r->state, (double *)r->comp, op->v64, md->coef64, [](double v) { return v; });
+    else
+        fill(r->a32, (float *)r->state, (float *)r->comp, op->v32, md->coef32, [](double v) { return (float)v; });
+
+    // one captured graph of guard_every steps followed by the energy guard
+    if (r->use_graph) {
+        hipGraph_t g = nullptr;
+        e = hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (int k = 0; k < r->guard_every && rc == SFG_OK; ++k) rc = launch_one_step(r);
+            if (rc == SFG_OK) rc = launch_guard_now(r);
+            hipError_t e2 = hipStreamEndCapture(r->stream, &g);
+            if (rc == SFG_OK && e2 != hipSuccess) rc = fail(SFG_E_HIP, "graph capture: %s", hipGetErrorString(e2));
+        } else {
+            rc = fail(SFG_E_HIP, "graph capture: %s", hipGetErrorString(e));
+        }
+        if (rc == SFG_OK) {
+            e = hipGraphInstantiate(&r->graph, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = fail(SFG_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != SFG_OK) {
+            run_free(r);
+            return rc;
+        }
+    }
+    *out = r;
+    return SFG_OK;
+}
+
+static int check_guard(sfg_run *r, int upto_step)
+{
+    const int nchk = std::min(r->nslots, (upto_step + r->guard_every - 1) / r->guard_every + 1);
+    std::vector<double> en((size_t)nchk * r->S * 2);
+    HIPCHK(hipMemcpy(en.data(), r->energy, sizeof(double) * en.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < nchk; ++c)
+        for (int s = 0; s < r->nshots; ++s) {
+            const double eu = en[((size_t)c * r->S + s) * 2], ew = en[((size_t)c * r->S + s) * 2 + 1];
+            if (!(eu <= 10e6) || !(ew <= 10e6))
+                return fail(SFG_E_ENERGY, "energy guard: shot %d sum(U^2)=%g sum(W^2)=%g at check %d (step <= %d)",
+                            r->perm[s], eu, ew, c, c * r->guard_every);
+        }
+    Ctrl h;
+    HIPCHK(hipMemcpy(&h, r->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    if (h.err & 1) return fail(SFG_E_ZERO_MASS, "zero lumped mass met (outputs of those nodes left unchanged)");
+    if (h.step != r->executed) return fail(SFG_E_HIP, "device step counter %d != host %d", h.step, r->executed);
+    return SFG_OK;
+}
+
+extern "C" int sfg_run_advance(sfg_run *r, int nsteps)
+{
+    g_err.clear();
+    if (!r || nsteps < 0) return fail(SFG_E_ARG, "advance: bad arguments");
+    int todo = std::min(nsteps, r->max_steps - r->executed);
+    const int steps = todo;
+    HIPCHK(hipEventRecord(r->ev0, r->stream));
+    while (todo > 0) {
+        if (r->graph && todo >= r->guard_every && r->executed % r->guard_every == 0) {
+            HIPCHK(hipGraphLaunch(r->graph, r->stream));
+            r->executed += r->guard_every;
+            todo -= r->guard_every;
+        } else {
+            int rc = launch_one_step(r);
+            if (rc) return rc;
+            r->executed += 1;
+            todo -= 1;
+            if (r->executed % r->guard_every == 0 || r->executed == r->max_steps) {
+                rc = launch_guard_now(r);
+                if (rc) return rc;
+            }
+        }
+    }
+    HIPCHK(hipEventRecord(r->ev1, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    r->last_ms = steps > 0 ? (double)ms / steps : 0.0;
+    int rc = check_guard(r, r->executed);
+    if (rc) return rc;
+    return r->executed;
+}
+
+extern "C" int sfg_run_sync(sfg_run *r)
+{
+    if (!r) return fail(SFG_E_ARG, "sync: null run");
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return r->executed;
+}
+
+extern "C" double sfg_run_last_ms_per_step(const sfg_run *r) { return r ? r->last_ms : 0.0; }
+
+extern "C" double sfg_run_step_bytes(const sfg_run *r, int active)
+{
+    if (!r) return 0.0;
+    const double N = r->op->n, nnz = r->op->nnz, w = r->prec == 0 ? 8.0 : 4.0;
+    const double K = r->prec == 2 ? 2.0 * 6.0 * w : 0.0;
+    return 4.0 * (N + 1) + nnz * (4.0 + 7.0 * w) + 11.0 * N * w + (double)active * N * (52.0 * w + K);
+}
+
+extern "C" int sfg_run_seismograms(sfg_run *r, double *u, double *w)
+{
+    g_err.clear();
+    if (!r || !u || !w) return fail(SFG_E_ARG, "seismograms: bad arguments");
+    HIPCHK(hipStreamSynchronize(r->stream));
+    const size_t per = (size_t)r->max_steps * r->rec_num;
+    if (per == 0) return SFG_OK;
+    std::vector<double> hu(per * r->S), hw(per * r->S);
+    HIPCHK(hipMemcpy(hu.data(), r->seis_u, sizeof(double) * hu.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hw.data(), r->seis_w, sizeof(double) * hw.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < r->nshots; ++s) {
+        std::memcpy(u + per * r->perm[s], hu.data() + per * s, sizeof(double) * per);
+        std::memcpy(w + per * r->perm[s], hw.data() + per * s, sizeof(double) * per);
+    }
+    return SFG_OK;
+}
+
+extern "C" int sfg_run_snapshot(sfg_run *r, int shot, double *U, double *W)
+{
+    g_err.clear();
+    if (!r || !U || !W || shot < 0 || shot >= r->nshots) return fail(SFG_E_ARG, "snapshot: bad arguments");
+    int s = 0;
+    while (r->perm[s] != shot) ++s;
+    // a shot that stopped early holds its state in the buffer of its own last step
+    const int done = std::min(r->executed, r->steps_int[s]);
+    const int n = r->op->n;
+    double *dU, *dW;
+    int rc = dalloc(&dU, 2 * (size_t)n);
+    if (rc) return rc;
+    dW = dU + n;
+    hipError_t e = launch_gather_uw(r->prec, r->state, n, r->S, (size_t)n * r->S, s, done & 1, dU, dW, r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e == hipSuccess) e = hipMemcpy(U, dU, sizeof(double) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(W, dW, sizeof(double) * n, hipMemcpyDeviceToHost);
+    dfree(dU);
+    if (e != hipSuccess) return fail(SFG_E_HIP, "snapshot: %s", hipGetErrorString(e));
+    return SFG_OK;
+}
+
+extern "C" int sfg_run_energy(sfg_run *r, double *energy, int max_checks)
+{
+    g_err.clear();
+    if (!r || !energy || max_checks <= 0) return fail(SFG_E_ARG, "energy: bad arguments");
+    HIPCHK(hipStreamSynchronize(r->stream));
+    const int nchk = std::min(max_checks, r->nslots);
+    std::vector<double> en((size_t)nchk * r->S * 2);
+    HIPCHK(hipMemcpy(en.data(), r->energy, sizeof(double) * en.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < nchk; ++c)
+        for (int s = 0; s < r->nshots; ++s)
+            for (int k = 0; k < 2; ++k)
+                energy[((size_t)c * r->nshots + r->perm[s]) * 2 + k] = en[((size_t)c * r->S + s) * 2 + k];
+    return nchk;
+}

@@ -1,0 +1,77 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace sfg {
+
+// Field order of the 26-array state of one split-field M-PML shot
+// (reference time_evolution/elastic_wave.c:78-87).
+enum Field : int {
+    U1 = 0, U2, U3, W1, W2, W3,         // split displacements
+    U1T, U2T, U3T, W1T, W2T, W3T,       // split velocities
+    U1TT, U2TT, U3TT, W1TT, W2TT, W3TT, // split accelerations (step n)
+    LX1, LX2, LX3, LX4, LY1, LY2, LY3, LY4, // M-PML memory fields
+    NFIELD
+};
+
+// Per-node coefficient order (c11,c13,c33,c44 from model_elastic_parameter.c:7,
+// M-PML profiles from abc_mpml.c:4, then the lumped mass term).
+enum Coef : int {
+    C11 = 0, C13, C33, C44, DX, DY, DXX, DYY, DXXP, DYYP, MASS, NCOEF
+};
+
+// Device-side control block: the step counter lets one captured graph replay
+// any chunk of steps; the last block of each step kernel advances it.
+struct Ctrl {
+    int step;           // number of executed steps
+    unsigned int done;  // blocks finished in the current step kernel
+    int err;            // bit 0: zero lumped mass met
+    int pad;
+};
+
+template <class T>
+struct StepArgs {
+    int n;                  // nodes (rows)
+    int S;                  // shot stride of the state (padded shot count)
+    int rows_per_block;
+    int nsteps_max;
+    const int *rp, *ci;     // union CSR pattern
+    const T *A[7];          // M, S1..S6 on the union pattern
+    const T *coef;          // NCOEF x n (x S when coefficients are per shot)
+    const int *flag;        // Dirichlet flags (exact mode), may be null otherwise
+    T *state;               // [2][NFIELD][n][S]
+    T *comp;                // [2][6][n][S] Kahan compensation of U1..W3 (or null)
+    size_t fstride;         // n * S
+    const int *active;      // active[t] = shots that execute step t (shots sorted)
+    const int *src;         // source node per shot (internal shot order)
+    const double *wavelet;  // Ricker value per step (time (t+3)*dt)
+    double src_u, src_w;    // force direction factors sin/cos(angle)
+    T dt, ca, alpha, cd, delta; // Newmark: ca = 0.5 - alpha, cd = 1 - delta
+    const int *rec_ptr, *rec_idx; // node -> receiver slots (CSR over nodes)
+    int rec_num;
+    double *seis_u, *seis_w; // [S][nsteps_max][rec_num]
+    Ctrl *ctrl;
+};
+
+// Launchers (defined in sfg_kernels.hip).  precision: 0 exact FP64, 1 FP32, 2 FP32+Kahan.
+hipError_t launch_step(int precision, int sp, bool coef_per_shot, const void *args, int nblk_x,
+                       int nblk_y, int block, hipStream_t st);
+hipError_t launch_guard(int precision, const void *state, int n, int S, size_t fstride,
+                        const Ctrl *ctrl, double *energy, int nslots, int every, hipStream_t st);
+hipError_t launch_gather_uw(int precision, const void *state, int n, int S, size_t fstride,
+                            int shot, int parity, double *U, double *W, hipStream_t st);
+hipError_t launch_spmv64(int n, const int *rp, const int *cj, const double *a, const double *x,
+                         double *y, hipStream_t st);
+// M-PML geometry: damping-layer edges, thicknesses, d0 and cross factors.
+struct MpmlGeo {
+    double x0, x1, y0, y1, tx, ty, d0x, d0y, pxy, pyx;
+    int use_xmin, use_xmax, use_ymin, use_ymax;
+};
+hipError_t launch_mpml(int n, const double *x, const double *y, const MpmlGeo &g, double *out6,
+                       hipStream_t st);
+hipError_t launch_to_f32(const double *src, float *dst, size_t n, hipStream_t st);
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t st);
+
+} // namespace sfg

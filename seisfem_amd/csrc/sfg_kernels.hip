@@ -1,0 +1,453 @@
+// HIP kernels for the explicit elastic M-PML masslump time loop (gfx950).
+//
+// One launch = one time step of every active shot.  Each lane owns one
+// (row, shot) pair: it walks the row of the union CSR pattern once, forms the
+// 32 distinct products of the reference's 44 csr_matvec calls (20 with the
+// consistent mass M, two with each of S1..S6), then does the row-local RHS,
+// lumped-mass divide + Dirichlet clamp, Newmark update and receiver store,
+// writing the other half of a double-buffered state.
+//
+// Lane mapping: SP lanes of a wavefront share one row and hold consecutive
+// shots (state layout [field][node][shot], shots innermost), so a wave covers
+// 64/SP rows.  SP = 1 is the single-shot path (one row per lane); SP = 64
+// gives one row per wave with the matrix row read once as scalars and
+// broadcast to 64 shots.
+//
+// Precision modes:
+//   0  FP64, bit-identical to the reference: every sum is sequential in
+//      column order, no FMA (the file is built with -ffp-contract=off), the
+//      RHS / divide / update keep the reference's operand order
+//      (time_evolution/elastic_wave.c:582-592, 648-658, 681-729, 754-778);
+//   1  FP32 storage and arithmetic (FMA in the dot products);
+//   2  FP32 with a Kahan-compensated displacement update.
+#include "sfg_internal.h"
+
+namespace sfg {
+
+template <bool EXACT, class T>
+__device__ __forceinline__ T mac(T acc, T a, T x)
+{
+    if constexpr (EXACT) return acc + a * x;
+    else return __builtin_fmaf(a, x, acc);
+}
+
+template <class T, int PREC, int SP, bool CPS>
+__global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a)
+{
+    constexpr bool EXACT = (PREC == 0);
+    constexpr bool KAHAN = (PREC == 2);
+    constexpr int RPW = (SP >= 64) ? 1 : 64 / SP;
+    const int t = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    int row = (blockIdx.x * wpb + wave) * RPW + (SP >= 64 ? 0 : lane / SP);
+    if constexpr (SP >= 64) row = __builtin_amdgcn_readfirstlane(row);
+    const int s = blockIdx.y * (SP >= 64 ? 64 : SP) + (SP >= 64 ? lane : lane % SP);
+    const int nact = (t < a.nsteps_max) ? a.active[t] : 0;
+
+    if (row < a.n && s < nact) {
+        const size_t F = a.fstride;
+        const int S = a.S;
+        const T *in = a.state + (size_t)(t & 1) * NFIELD * F;
+        T *out = a.state + (size_t)((t + 1) & 1) * NFIELD * F;
+
+        // ---- the 32 distinct products over one walk of the row ------------
+        T s1u = 0, s1w = 0, s2u = 0, s2w = 0, s3u = 0, s3w = 0;
+        T s4u = 0, s4w = 0, s5u = 0, s5w = 0, s6u = 0, s6w = 0;
+        T mu1t = 0, mu1 = 0, mlx1 = 0, mlx2 = 0, mu2t = 0, mu2 = 0, mu3t = 0, mu3 = 0, mlx3 = 0, mlx4 = 0;
+        T mw1t = 0, mw1 = 0, mly1 = 0, mly2 = 0, mw2t = 0, mw2 = 0, mw3t = 0, mw3 = 0, mly3 = 0, mly4 = 0;
+        const int k0 = a.rp[row], k1 = a.rp[row + 1];
+        for (int k = k0; k < k1; ++k) {
+            const int j = a.ci[k];
+            const T am = a.A[0][k];
+            const T a1 = a.A[1][k], a2 = a.A[2][k], a3 = a.A[3][k];
+            const T a4 = a.A[4][k], a5 = a.A[5][k], a6 = a.A[6][k];
+            const T *x = in + (size_t)j * S + s;
+            const T u1 = x[U1 * F], u2 = x[U2 * F], u3 = x[U3 * F];
+            const T w1 = x[W1 * F], w2 = x[W2 * F], w3 = x[W3 * F];
+            const T u1t = x[U1T * F], u2t = x[U2T * F], u3t = x[U3T * F];
+            const T w1t = x[W1T * F], w2t = x[W2T * F], w3t = x[W3T * F];
+            const T lx1 = x[LX1 * F], lx2 = x[LX2 * F], lx3 = x[LX3 * F], lx4 = x[LX4 * F];
+            const T ly1 = x[LY1 * F], ly2 = x[LY2 * F], ly3 = x[LY3 * F], ly4 = x[LY4 * F];
+            // U_now / W_now as recombined at the end of the previous step (:775-776)
+            const T U = u1 + u2 + u3;
+            const T W = w1 + w2 + w3;
+            s1u = mac<EXACT>(s1u, a1, U); s1w = mac<EXACT>(s1w, a1, W);
+            s2u = mac<EXACT>(s2u, a2, U); s2w = mac<EXACT>(s2w, a2, W);
+            s3u = mac<EXACT>(s3u, a3, U); s3w = mac<EXACT>(s3w, a3, W);
+            s4u = mac<EXACT>(s4u, a4, U); s4w = mac<EXACT>(s4w, a4, W);
+            s5u = mac<EXACT>(s5u, a5, U); s5w = mac<EXACT>(s5w, a5, W);
+            s6u = mac<EXACT>(s6u, a6, U); s6w = mac<EXACT>(s6w, a6, W);
+            mu1t = mac<EXACT>(mu1t, am, u1t); mu1 = mac<EXACT>(mu1, am, u1);
+            mlx1 = mac<EXACT>(mlx1, am, lx1); mlx2 = mac<EXACT>(mlx2, am, lx2);
+            mu2t = mac<EXACT>(mu2t, am, u2t); mu2 = mac<EXACT>(mu2, am, u2);
+            mu3t = mac<EXACT>(mu3t, am, u3t); mu3 = mac<EXACT>(mu3, am, u3);
+            mlx3 = mac<EXACT>(mlx3, am, lx3); mlx4 = mac<EXACT>(mlx4, am, lx4);
+            mw1t = mac<EXACT>(mw1t, am, w1t); mw1 = mac<EXACT>(mw1, am, w1);
+            mly1 = mac<EXACT>(mly1, am, ly1); mly2 = mac<EXACT>(mly2, am, ly2);
+            mw2t = mac<EXACT>(mw2t, am, w2t); mw2 = mac<EXACT>(mw2, am, w2);
+            mw3t = mac<EXACT>(mw3t, am, w3t); mw3 = mac<EXACT>(mw3, am, w3);
+            mly3 = mac<EXACT>(mly3, am, ly3); mly4 = mac<EXACT>(mly4, am, ly4);
+        }
+
+        // ---- per-node coefficients --------------------------------------
+        const size_t cs = CPS ? (size_t)a.n * S : (size_t)a.n;
+        const size_t ci = CPS ? (size_t)row * S + s : (size_t)row;
+        const T c11 = a.coef[C11 * cs + ci], c13 = a.coef[C13 * cs + ci];
+        const T c33 = a.coef[C33 * cs + ci], c44 = a.coef[C44 * cs + ci];
+        const T dx = a.coef[DX * cs + ci], dy = a.coef[DY * cs + ci];
+        const T dxx = a.coef[DXX * cs + ci], dyy = a.coef[DYY * cs + ci];
+        const T dxxp = a.coef[DXXP * cs + ci], dyyp = a.coef[DYYP * cs + ci];
+        const T mass = a.coef[MASS * cs + ci];
+        const T dt = a.dt;
+
+        // point force, (i == source_node) * point_source * sin|cos(angle)
+        const double is_src = (row == a.src[s]) ? 1.0 : 0.0;
+        const double ps = a.wavelet[t];
+        const T fu = (T)(is_src * ps * a.src_u);
+        const T fw = (T)(is_src * ps * a.src_w);
+
+        // ---- RHS of the 14 split equations (operand order of :585-591, :651-657)
+        const T ru1 = -c11 * s1u - T(2.0) * dx * mu1t - dx * dx * mu1 + mlx1 + mlx2 + fu;
+        const T ru2 = -c13 * s3w - c44 * s4w - dx * mu2t - dy * mu2t - dx * dy * mu2;
+        const T ru3 = -c44 * s2u - T(2.0) * dy * mu3t - dy * dy * mu3 + mlx3 + mlx4;
+        const T ru4 = -dt * c11 * dxx * s5u - dt * dx * mlx1 + mlx1;
+        const T ru5 = -dt * c44 * dyyp * s5w - dt * dy * mlx2 + mlx2;
+        const T ru6 = -dt * c13 * dxxp * s6w - dt * dx * mlx3 + mlx3;
+        const T ru7 = -dt * c44 * dyy * s6u - dt * dy * mlx4 + mlx4;
+        const T rw1 = -c44 * s1w - T(2.0) * dx * mw1t - dx * dx * mw1 + mly1 + mly2 + fw;
+        const T rw2 = -c44 * s3u - c13 * s4u - dx * mw2t - dy * mw2t - dx * dy * mw2;
+        const T rw3 = -c33 * s2w - T(2.0) * dy * mw3t - dy * dy * mw3 + mly3 + mly4;
+        const T rw4 = -dt * c44 * dxx * s5w - dt * dx * mly1 + mly1;
+        const T rw5 = -dt * c13 * dyyp * s5u - dt * dy * mly2 + mly2;
+        const T rw6 = -dt * c44 * dxxp * s6u - dt * dx * mly3 + mly3;
+        const T rw7 = -dt * c33 * dyy * s6w - dt * dy * mly4 + mly4;
+
+        // ---- own-row state ----------------------------------------------
+        const size_t o = (size_t)row * S + s;
+        T cur[NFIELD];
+#pragma unroll
+        for (int f = 0; f < NFIELD; ++f) cur[f] = in[f * F + o];
+
+        // ---- lumped-mass solve with Dirichlet clamp (:681-729) ------------
+        T an[6], ln[8];
+        if constexpr (EXACT) {
+            if (mass != T(0)) {
+                if (a.flag[row] == 1) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) an[q] = T(0);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) ln[q] = T(0);
+                } else {
+                    const T m1 = T(1.0) * mass;
+                    an[0] = ru1 / m1; an[1] = ru2 / m1; an[2] = ru3 / m1;
+                    ln[0] = ru4 / m1; ln[1] = ru5 / m1; ln[2] = ru6 / m1; ln[3] = ru7 / m1;
+                    an[3] = rw1 / m1; an[4] = rw2 / m1; an[5] = rw3 / m1;
+                    ln[4] = rw4 / m1; ln[5] = rw5 / m1; ln[6] = rw6 / m1; ln[7] = rw7 / m1;
+                }
+            } else {
+                // zero lumped mass: the reference leaves the outputs untouched
+#pragma unroll
+                for (int q = 0; q < 6; ++q) an[q] = cur[U1TT + q];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) ln[q] = cur[LX1 + q];
+                atomicOr(&a.ctrl->err, 1);
+            }
+        } else {
+            // mass holds 1/mass_lump with the Dirichlet flag folded in as 0
+            an[0] = ru1 * mass; an[1] = ru2 * mass; an[2] = ru3 * mass;
+            ln[0] = ru4 * mass; ln[1] = ru5 * mass; ln[2] = ru6 * mass; ln[3] = ru7 * mass;
+            an[3] = rw1 * mass; an[4] = rw2 * mass; an[5] = rw3 * mass;
+            ln[4] = rw4 * mass; ln[5] = rw5 * mass; ln[6] = rw6 * mass; ln[7] = rw7 * mass;
+        }
+
+        // ---- Newmark update (:757-776) ---------------------------------------
+        const T ca = a.ca, al = a.alpha, cd = a.cd, de = a.delta;
+        T dn[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const T d = cur[U1 + q], v = cur[U1T + q], acc = cur[U1TT + q];
+            if constexpr (KAHAN) {
+                const size_t C6 = (size_t)6 * F;
+                const T *cin = a.comp + (size_t)(t & 1) * C6;
+                T *cout = a.comp + (size_t)((t + 1) & 1) * C6;
+                const T inc = v * dt + (ca * acc + al * an[q]) * dt * dt;
+                const T y = inc - cin[q * F + o];
+                const T sum = d + y;
+                cout[q * F + o] = (sum - d) - y;
+                dn[q] = sum;
+            } else {
+                dn[q] = d + v * dt + (ca * acc + al * an[q]) * dt * dt;
+            }
+            out[(U1T + q) * F + o] = v + (cd * acc + de * an[q]) * dt;
+            out[(U1TT + q) * F + o] = an[q];
+            out[(U1 + q) * F + o] = dn[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[(LX1 + q) * F + o] = ln[q];
+
+        // ---- receivers sitting on this node --------------------------------
+        const int r0 = a.rec_ptr[row], r1 = a.rec_ptr[row + 1];
+        if (r0 < r1) {
+            const double Un = (double)(dn[0] + dn[1] + dn[2]);
+            const double Wn = (double)(dn[3] + dn[4] + dn[5]);
+            for (int q = r0; q < r1; ++q) {
+                const size_t w = ((size_t)s * a.nsteps_max + t) * a.rec_num + a.rec_idx[q];
+                a.seis_u[w] = Un;
+                a.seis_w[w] = Wn;
+            }
+        }
+    }
+
+    // ---- step completion: the last block advances the step counter ----------
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int total = gridDim.x * gridDim.y;
+        __threadfence();
+        const unsigned int prev = atomicAdd(&a.ctrl->done, 1u);
+        if (prev == total - 1) {
+            __hip_atomic_store(&a.ctrl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.ctrl->step, t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <class T, int PREC>
+static hipError_t launch_step_t(int sp, bool cps, const void *args, dim3 g, int block, hipStream_t st)
+{
+    const StepArgs<T> &a = *static_cast<const StepArgs<T> *>(args);
+#define SFG_L(SPV, C) hipLaunchKernelGGL((step_kernel<T, PREC, SPV, C>), g, dim3(block), 0, st, a)
+    switch (sp) {
+    case 1: if (cps) SFG_L(1, true); else SFG_L(1, false); break;
+    case 4: if (cps) SFG_L(4, true); else SFG_L(4, false); break;
+    case 16: if (cps) SFG_L(16, true); else SFG_L(16, false); break;
+    case 64: if (cps) SFG_L(64, true); else SFG_L(64, false); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef SFG_L
+    return hipGetLastError();
+}
+
+hipError_t launch_step(int precision, int sp, bool cps, const void *args, int nbx, int nby,
+                       int block, hipStream_t st)
+{
+    const dim3 g(nbx, nby);
+    switch (precision) {
+    case 0: return launch_step_t<double, 0>(sp, cps, args, g, block, st);
+    case 1: return launch_step_t<float, 1>(sp, cps, args, g, block, st);
+    case 2: return launch_step_t<float, 2>(sp, cps, args, g, block, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---- energy guard: sum(U^2), sum(W^2) per shot (reference :777-785) ----------
+// Threads are laid out [rows][shots] (shots innermost, as the state), each
+// accumulates a strided set of nodes, then a block reduction per shot.
+template <class T>
+__global__ __launch_bounds__(256) void guard_kernel(const T *state, int n, int S, size_t F,
+                                                    const Ctrl *ctrl, double *energy, int nslots,
+                                                    int every, int nodes_per_block)
+{
+    __shared__ double red_u[256], red_w[256];
+    const int t = __hip_atomic_load(&ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int sg = S < 64 ? S : 64;                 // shots covered by one block
+    const int rstep = blockDim.x / sg;
+    const int sl = threadIdx.x % sg, rl = threadIdx.x / sg;
+    const int s = blockIdx.y * sg + sl;
+    const T *in = state + (size_t)(t & 1) * NFIELD * F;
+    double eu = 0.0, ew = 0.0;
+    const int n0 = blockIdx.x * nodes_per_block;
+    const int n1 = min(n, n0 + nodes_per_block);
+    if (rl < rstep) {
+        for (int i = n0 + rl; i < n1; i += rstep) {
+            const size_t o = (size_t)i * S + s;
+            const double U = (double)(in[U1 * F + o] + in[U2 * F + o] + in[U3 * F + o]);
+            const double W = (double)(in[W1 * F + o] + in[W2 * F + o] + in[W3 * F + o]);
+            eu += U * U;
+            ew += W * W;
+        }
+    }
+    red_u[threadIdx.x] = eu;
+    red_w[threadIdx.x] = ew;
+    __syncthreads();
+    if (rl == 0) {
+        for (int q = 1; q < rstep; ++q) {
+            eu += red_u[q * sg + sl];
+            ew += red_w[q * sg + sl];
+        }
+        // check c covers the steps ((c-1)*every, c*every]
+        const int slot = min((t + every - 1) / every, nslots - 1);
+        atomicAdd(&energy[((size_t)slot * S + s) * 2 + 0], eu);
+        atomicAdd(&energy[((size_t)slot * S + s) * 2 + 1], ew);
+    }
+}
+
+hipError_t launch_guard(int precision, const void *state, int n, int S, size_t F, const Ctrl *ctrl,
+                        double *energy, int nslots, int every, hipStream_t st)
+{
+    const int npb = 2048;
+    const dim3 g((n + npb - 1) / npb, S < 64 ? 1 : S / 64);
+    if (precision == 0)
+        hipLaunchKernelGGL(guard_kernel<double>, g, dim3(256), 0, st, (const double *)state, n, S, F,
+                           ctrl, energy, nslots, every, npb);
+    else
+        hipLaunchKernelGGL(guard_kernel<float>, g, dim3(256), 0, st, (const float *)state, n, S, F,
+                           ctrl, energy, nslots, every, npb);
+    return hipGetLastError();
+}
+
+// ---- U = U1+U2+U3, W = W1+W2+W3 of one shot (snapshots, :775-776, :797-806) --
+template <class T>
+__global__ void gather_uw_kernel(const T *state, int n, int S, size_t F, int shot, int parity,
+                                 double *U, double *W)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T *in = state + (size_t)parity * NFIELD * F;
+    const size_t o = (size_t)i * S + shot;
+    U[i] = (double)(in[U1 * F + o] + in[U2 * F + o] + in[U3 * F + o]);
+    W[i] = (double)(in[W1 * F + o] + in[W2 * F + o] + in[W3 * F + o]);
+}
+
+hipError_t launch_gather_uw(int precision, const void *state, int n, int S, size_t F, int shot,
+                            int parity, double *U, double *W, hipStream_t st)
+{
+    const dim3 g((n + 255) / 256);
+    if (precision == 0)
+        hipLaunchKernelGGL(gather_uw_kernel<double>, g, dim3(256), 0, st, (const double *)state, n, S,
+                           F, shot, parity, U, W);
+    else
+        hipLaunchKernelGGL(gather_uw_kernel<float>, g, dim3(256), 0, st, (const float *)state, n, S,
+                           F, shot, parity, U, W);
+    return hipGetLastError();
+}
+
+// ---- y = A x, one row per lane, sequential row sums (csr_matvec.c:8-17) -------
+__global__ void spmv64_kernel(int n, const int *rp, const int *cj, const double *a, const double *x,
+                              double *y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double t = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) t = t + a[k] * x[cj[k]];
+    y[i] = t;
+}
+
+hipError_t launch_spmv64(int n, const int *rp, const int *cj, const double *a, const double *x,
+                         double *y, hipStream_t st)
+{
+    hipLaunchKernelGGL(spmv64_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, rp, cj, a, x, y);
+    return hipGetLastError();
+}
+
+// ---- M-PML damping profiles in FP64 (pml/abc_mpml.c:50-199) ---------------------
+// out6 = 6 arrays of n: dx, dy, dxx, dyy, dxx_pyx, dyy_pxy.  The region tests
+// and expressions keep the reference's order; every node gets its value from
+// its own coordinates, which is what the reference's per-element visits give.
+__global__ void mpml_kernel(int n, const double *X, const double *Y, const MpmlGeo g, double *out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double x0 = g.x0, x1 = g.x1, y0 = g.y0, y1 = g.y1;
+    const double tx = g.tx, ty = g.ty, d0x = g.d0x, d0y = g.d0y;
+    const double pxy = g.pxy, pyx = g.pyx;
+    const int uxn = g.use_xmin, uxx = g.use_xmax, uyn = g.use_ymin, uyx = g.use_ymax;
+    const double x = X[p], y = Y[p];
+    double dx = 0.0, dy = 0.0, dxx = 0.0, dyy = 0.0;
+    if ((x <= x0) && (y >= y0) && (y <= y1) && (uxn == 1)) {          // left strip
+        dx = d0x * ((x0 - x) / tx) * ((x0 - x) / tx);
+        dy = pyx * dx;
+        dxx = -d0x * 2 * (x0 - x) / (tx * tx);
+        dyy = 0.0;
+    } else if ((x >= x1) && (y >= y0) && (y <= y1) && (uxx == 1)) {   // right strip
+        dx = d0x * ((x - x1) / tx) * ((x - x1) / tx);
+        dy = pyx * dx;
+        dxx = d0x * 2 * (x - x1) / (tx * tx);
+        dyy = 0.0;
+    } else if ((y <= y0) && (x >= x0) && (x <= x1) && (uyn == 1)) {   // low-y strip
+        dy = d0y * ((y0 - y) / ty) * ((y0 - y) / ty);
+        dx = pxy * dy;
+        dxx = 0.0;
+        dyy = -d0y * 2 * (y0 - y) / (ty * ty);
+    } else if ((y >= y1) && (x >= x0) && (x <= x1) && (uyx == 1)) {   // high-y strip
+        dy = d0y * ((y - y1) / ty) * ((y - y1) / ty);
+        dx = pxy * dy;
+        dxx = 0.0;
+        dyy = d0y * 2 * (y - y1) / (ty * ty);
+    } else if ((x <= x0) && (y <= y0) && ((uxn == 1) || (uyn == 1))) { // corner 12
+        dx = d0x * ((x0 - x) / tx) * ((x0 - x) / tx) + pxy * d0y * ((y0 - y) / ty) * ((y0 - y) / ty);
+        dy = d0y * ((y0 - y) / ty) * ((y0 - y) / ty) + pyx * d0x * ((x0 - x) / tx) * ((x0 - x) / tx);
+        dxx = -d0x * 2 * (x0 - x) / (tx * tx);
+        dyy = -d0y * 2 * (y0 - y) / (ty * ty);
+    } else if ((x <= x0) && (y >= y1) && ((uxn == 1) || (uyx == 1))) { // corner 14
+        dx = d0x * ((x0 - x) / tx) * ((x0 - x) / tx) + pxy * d0y * ((y - y1) / ty) * ((y - y1) / ty);
+        dy = d0y * ((y - y1) / ty) * ((y - y1) / ty) + pyx * d0x * ((x0 - x) / tx) * ((x0 - x) / tx);
+        dyy = d0y * 2 * (y - y1) / (ty * ty);
+        dxx = -d0x * 2 * (x0 - x) / (tx * tx);
+    } else if ((x >= x1) && (y <= y0) && ((uxx == 1) || (uyn == 1))) { // corner 23
+        if (uyx == 1) { // the reference tests the ymax switch here (abc_mpml.c:147)
+            dx = d0x * ((x - x1) / tx) * ((x - x1) / tx) + pxy * d0y * ((y0 - y) / ty) * ((y0 - y) / ty);
+            dxx = d0x * 2 * (x - x1) / (tx * tx);
+            dy = d0y * ((y0 - y) / ty) * ((y0 - y) / ty) + pyx * d0x * ((x - x1) / tx) * ((x - x1) / tx);
+        } else {
+            dx = pxy * d0y * ((y0 - y) / ty) * ((y0 - y) / ty);
+            dxx = 0.0;
+            dy = d0y * ((y0 - y) / ty) * ((y0 - y) / ty);
+        }
+        dyy = -d0y * 2 * (y0 - y) / (ty * ty);
+    } else if ((x >= x1) && (y >= y1) && ((uxx == 1) || (uyx == 1))) { // corner 34
+        if (uyx == 1) {
+            dx = d0x * ((x - x1) / tx) * ((x - x1) / tx) + pxy * d0y * ((y - y1) / ty) * ((y - y1) / ty);
+            dy = d0y * ((y - y1) / ty) * ((y - y1) / ty) + pyx * d0x * ((x - x1) / tx) * ((x - x1) / tx);
+            dxx = d0x * 2 * (x - x1) / (tx * tx);
+        } else {
+            dx = pxy * d0y * ((y - y1) / ty) * ((y - y1) / ty);
+            dy = d0y * ((y - y1) / ty) * ((y - y1) / ty);
+            dxx = 0.0;
+        }
+        dyy = d0y * 2 * (y - y1) / (ty * ty);
+    }
+    out[0 * (size_t)n + p] = dx;
+    out[1 * (size_t)n + p] = dy;
+    out[2 * (size_t)n + p] = dxx;
+    out[3 * (size_t)n + p] = dyy;
+    out[4 * (size_t)n + p] = pyx * dxx;
+    out[5 * (size_t)n + p] = pxy * dyy;
+}
+
+hipError_t launch_mpml(int n, const double *x, const double *y, const MpmlGeo &g, double *out6,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(mpml_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, x, y, g, out6);
+    return hipGetLastError();
+}
+
+__global__ void to_f32_kernel(const double *src, float *dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
+hipError_t launch_to_f32(const double *src, float *dst, size_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(to_f32_kernel, dim3(1024), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+
+__global__ void fill_zero_kernel(uint32_t *p, size_t nwords)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t st)
+{
+    // every buffer here is a multiple of 4 bytes
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, st, (uint32_t *)p, bytes / 4);
+    return hipGetLastError();
+}
+
+} // namespace sfg

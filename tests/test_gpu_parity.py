@@ -1,0 +1,166 @@
+"""GPU parity: the HIP time loop against the oracle, through the C ABI.
+
+Bar: FP64 results are bit-identical to the oracle (which reproduces the
+reference's shipped outputs); FP32 and FP32+Kahan runs stay within a stated
+relative-L2 tolerance of the FP64 oracle.  Every case prints its name before
+touching the GPU.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from seisfem_amd import formats, gpu, host, workloads
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(msg):
+    print("[gpu-case]", msg, flush=True)
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_csr_matvec_dropin_bit_identical():
+    en, x, y = host.mesh_structured(host.Q9, 17, 13, 0.0, 170.0, 0.0, 130.0)
+    ci, cj, vals = host.assemble_coo(host.Q9, en, x, y, np.random.default_rng(0).uniform(1e3, 3e3, x.size))
+    lib = __import__("seisfem_amd").lib()
+    for m in range(7):
+        p, j, v = host.coo2csr(x.size, ci, cj, vals[m])
+        xv = np.random.default_rng(m).standard_normal(x.size)
+        say(f"csr_matvec Q9 17x13 matrix {host.MATRIX_NAMES[m]} n={x.size}")
+        yg = np.zeros(x.size)
+        ip = lambda a: a.ctypes.data_as(__import__("ctypes").POINTER(__import__("ctypes").c_int))
+        dp = lambda a: a.ctypes.data_as(__import__("ctypes").POINTER(__import__("ctypes").c_double))
+        lib.csr_matvec(p.size, ip(p), ip(j), dp(v), dp(xv), dp(yg))
+        assert __import__("seisfem_amd").last_error() == ""
+        assert bits_equal(yg, oracle.oracle_csr_matvec(p, j, v, xv))
+
+
+@pytest.mark.parametrize("tc,nx,ny,pml,free", [(host.Q4, 30, 24, 6, 1), (host.Q9, 12, 10, 3, 0),
+                                               (host.T6, 14, 9, 4, 1), (host.T3, 20, 30, 5, 0)])
+def test_abc_mpml_dropin_bit_identical(tc, nx, ny, pml, free):
+    edge = 10.0
+    en, x, y = host.mesh_structured(tc, nx, ny, 0.0, edge * nx, 0.0, edge * ny)
+    o = host.element_order(tc)
+    use = (1, 1 - free, 1, 1)
+    args = (en, o, x, y, pml, pml + 1, edge, 0.0, edge * nx, 0.0, edge * ny, 3210.5, use)
+    say(f"abc_mpml {host.TYPE_NAMES[tc]} {nx}x{ny}")
+    g = host.abc_mpml(*args)
+    o_ = oracle.oracle_abc_mpml(*args)
+    for a, b in zip(g, o_):
+        assert bits_equal(a, b)
+
+
+def _case(tc, nelem, steps, nshots=1):
+    return workloads.small_q4(nelem=nelem, steps=steps, nshots=nshots, type_code=tc)
+
+
+@pytest.mark.parametrize("tc,nelem,steps,graph,guard", [
+    (host.Q4, 20, 150, True, 50), (host.Q4, 20, 150, False, 7), (host.Q9, 10, 120, True, 13),
+    (host.T6, 10, 120, True, 50), (host.T3, 20, 120, False, 50)])
+def test_fp64_loop_bit_identical(tc, nelem, steps, graph, guard):
+    case = _case(tc, nelem, steps)
+    prob = case["problem"]
+    rec = np.arange(prob["node_num"], dtype=np.int32)  # every node is a receiver
+    say(f"FP64 loop {host.TYPE_NAMES[tc]} {nelem}x{nelem} steps={steps} graph={graph} guard={guard}")
+    res = gpu.simulate(prob, case["src"], steps, rec, case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
+                       use_graph=graph, guard_every=guard)
+    ref = oracle.oracle_masslump(prob, case["src"][0], steps + 2, rec, case["dt"], case["f0"], case["t0"])
+    assert ref["status"] == steps
+    assert np.max(np.abs(ref["seis_u"])) > 0
+    assert bits_equal(res["seis_u"][0], ref["seis_u"])
+    assert bits_equal(res["seis_w"][0], ref["seis_w"])
+
+
+def test_example6_on_gpu_character_identical():
+    from test_oracle import ex6_problem
+    par, prob, src, rec = ex6_problem()
+    # the GPU computes its own M-PML profiles here
+    prob = dict(prob)
+    xmin, xmax, ymin, ymax = prob["bbox"]
+    prob["mpml"] = host.abc_mpml(prob["element_node"], 3, prob["x"], prob["y"], 20, 20, par["edge_size"],
+                                 xmin, xmax, ymin, ymax, prob["vp_max"], prob["use_mpml"])
+    say("example 6 T3 23225 nodes, 1198 steps, FP64")
+    res = gpu.simulate(prob, src, par["step"] - 2, rec, par["dt"], par["f0"], par["t0"], precision=gpu.FP64,
+                       snap_every=200)
+    out = os.path.join(GOLD, "ex6", "outputfile")
+    assert formats.format_rows(res["seis_u"][0]) == formats.read_text(os.path.join(out, "seismogram_u_shot_1.txt"))
+    assert formats.format_rows(res["seis_w"][0]) == formats.read_text(os.path.join(out, "seismogram_w_shot_1.txt"))
+    assert formats.format_rows(res["snaps_u"], formats.SNAP_SEP) == formats.read_text(
+        os.path.join(out, "wavefield_u_shot_1.txt"))
+    assert formats.format_rows(res["snaps_w"], formats.SNAP_SEP) == formats.read_text(
+        os.path.join(out, "wavefield_w_shot_1.txt"))
+
+
+@pytest.mark.parametrize("nshots,sp", [(3, 4), (11, 16), (70, 64)])
+def test_multishot_fp64_each_shot_bit_identical(nshots, sp):
+    case = _case(host.Q4, 14, 90, nshots=nshots)
+    prob = case["problem"]
+    rng = np.random.default_rng(nshots)
+    nsteps = rng.integers(40, 91, nshots).astype(np.int32)
+    nsteps[0] = 90
+    say(f"multi-shot FP64 Q4 14x14 shots={nshots} sp={sp} uneven steps")
+    op = gpu.Operator(prob["node_num"], prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    run = gpu.Run(op, md, case["src"], 90, case["rec"], case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
+                  nsteps=nsteps, shots_per_lane_group=sp, guard_every=25)
+    run.advance(90)
+    u, w = run.seismograms()
+    check = sorted(set([0, nshots - 1, nshots // 2, int(np.argmin(nsteps))]))
+    for s in check:
+        ref = oracle.oracle_masslump(prob, case["src"][s], int(nsteps[s]) + 2, case["rec"], case["dt"], case["f0"],
+                                     case["t0"], snap_every=0)
+        k = int(nsteps[s])
+        assert bits_equal(u[s, :k], ref["seis_u"]) and bits_equal(w[s, :k], ref["seis_w"]), s
+        assert not np.any(u[s, k:])
+        U, W = run.snapshot(s)
+        refU = oracle.oracle_masslump(prob, case["src"][s], k + 2, np.arange(prob["node_num"], dtype=np.int32),
+                                      case["dt"], case["f0"], case["t0"])["seis_u"][-1]
+        assert bits_equal(U, refU)
+    run.close()
+    md.close()
+    op.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(gpu.FP32, 1e-4), (gpu.FP32_KAHAN, 1e-4)])
+def test_fp32_within_tolerance_of_fp64_oracle(prec, tol):
+    """FP32 seismograms within rel-L2 1e-4 of the FP64 oracle (the stated bar)."""
+    case = _case(host.Q4, 48, 900)
+    prob = case["problem"]
+    say(f"FP32 precision={prec} Q4 48x48 900 steps")
+    res = gpu.simulate(prob, case["src"], 900, case["rec"], case["dt"], case["f0"], case["t0"], precision=prec)
+    ref = oracle.oracle_masslump(prob, case["src"][0], 902, case["rec"], case["dt"], case["f0"], case["t0"])
+    su = np.concatenate([res["seis_u"][0].ravel(), res["seis_w"][0].ravel()])
+    sr = np.concatenate([ref["seis_u"].ravel(), ref["seis_w"].ravel()])
+    err = rel_l2(su, sr)
+    say(f"  rel-L2 error {err:.3e}")
+    assert err < tol
+
+
+def test_energy_guard_reports_instead_of_exiting():
+    """dt far above the stability limit: the run stops with SFG_E_ENERGY
+    (the reference calls exit(1)); kernels finish normally on inf/NaN."""
+    case = _case(host.Q4, 12, 400)
+    prob = case["problem"]
+    say("energy guard Q4 12x12 dt x8")
+    op = gpu.Operator(prob["node_num"], prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    run = gpu.Run(op, md, case["src"], 400, case["rec"], case["dt"] * 8, case["f0"], case["t0"],
+                  precision=gpu.FP64, guard_every=20)
+    with pytest.raises(__import__("seisfem_amd").NativeError) as ei:
+        run.advance(400)
+    assert ei.value.code == -6
+    run.close()
+    md.close()
+    op.close()

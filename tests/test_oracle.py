@@ -1,0 +1,163 @@
+"""The oracle (C restatement of the reference loop) against the reference.
+
+  * abc_mpml, csr_matvec and seismic_source: bit-identical to the reference's
+    own compiled sources (oracle/_ref) where those are built;
+  * the whole masslump loop: example 6 of the reference (T3 exterior mesh,
+    free surface, 1198 steps, 81 receivers) reproduces the shipped seismograms
+    and wavefield snapshots character for character at the reference's %f
+    output; example 3 (T6, internal mesh) reproduces its shipped seismograms;
+  * closed forms and invariants: Ricker values, zero source -> zero field,
+    linearity in the source amplitude, homogeneous-medium symmetry.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from seisfem_amd import formats, host, workloads
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+
+
+def _oracle_mpml(*a, **k):
+    return oracle.oracle_abc_mpml(*a, **k)
+
+
+@needs_ref
+@pytest.mark.parametrize("tc,nx,ny,pml,free", [(host.Q4, 30, 24, 6, 1), (host.Q9, 12, 10, 3, 0),
+                                               (host.T6, 14, 9, 4, 1), (host.T3, 20, 30, 5, 0)])
+def test_abc_mpml_bit_identical_to_reference(tc, nx, ny, pml, free):
+    edge = 10.0
+    en, x, y = host.mesh_structured(tc, nx, ny, 0.0, edge * nx, 0.0, edge * ny)
+    o = host.element_order(tc)
+    use = (1, 1 - free, 1, 1)
+    args = (en, o, x, y, pml, pml + 1, edge, 0.0, edge * nx, 0.0, edge * ny, 3210.5, use)
+    mine = oracle.oracle_abc_mpml(*args)
+    ref = oracle.ref_abc_mpml(*args)
+    for a, b in zip(mine, ref):
+        assert np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+@needs_ref
+def test_csr_matvec_and_ricker_bit_identical_to_reference():
+    en, x, y = host.mesh_structured(host.Q9, 9, 7, 0.0, 90.0, 0.0, 70.0)
+    ci, cj, vals = host.assemble_coo(host.Q9, en, x, y, np.full(x.size, 2000.0))
+    p, j, v = host.coo2csr(x.size, ci, cj, vals[3])
+    xv = np.random.default_rng(3).standard_normal(x.size)
+    y1 = oracle.oracle_csr_matvec(p, j, v, xv)
+    y2 = np.zeros(x.size)
+    L = oracle.rlib()
+    L.csr_matvec(p.size, oracle._ip(p), oracle._ip(j), oracle._dp(v), oracle._dp(xv), oracle._dp(y2))
+    assert np.array_equal(y1.view(np.int64), y2.view(np.int64))
+    for t in np.linspace(0.0, 0.3, 37):
+        assert oracle.olib().oracle_seismic_source(12.0, 0.1, 1e10, t) == L.seismic_source(12.0, 0.1, 1e10, t)
+
+
+def test_ricker_closed_form():
+    f = oracle.olib().oracle_seismic_source
+    assert f(10.0, 0.1, 1e10, 0.1) == 1e10                   # peak at t = t0
+    tz = 0.1 + np.sqrt(0.5) / (np.pi * 10.0)                 # zero crossing, a = 1/2
+    assert abs(f(10.0, 0.1, 1e10, tz)) < 1e-3
+    a, b = f(10.0, 0.1, 1e10, 0.1 + 0.05), f(10.0, 0.1, 1e10, 0.1 - 0.05)   # even about t0
+    assert abs(a - b) <= 1e-12 * abs(a)
+
+
+def _ex_problem(dirname):
+    par = formats.read_par(os.path.join(dirname, "par.txt"))
+    tc = par["type_code"]
+    o = host.element_order(tc)
+    edge = par["edge_size"]
+    if par["use_exterior_mesh"]:
+        en, x, y, bbox = formats.read_exterior_mesh(os.path.join(dirname, "mesh"), o)
+        rho, vp, vs, c = formats.read_velocity(os.path.join(dirname, "mesh", "velocity_and_density.txt"), x.size)
+    else:
+        nx, ny = par["nelemx"], par["nelemy"]
+        bbox = (0.0, edge * nx, 0.0, edge * ny)
+        en, x, y = host.mesh_structured(tc, nx, ny, *bbox)
+        # the reference's built-in homogeneous model (model_elastic_parameter.c:34-45)
+        rho, vp, vs = np.full(x.size, 2200.0), np.full(x.size, 2000.0), np.full(x.size, 1154.7)
+        c = host.elastic_constants(rho, vp, vs)
+    use = (1, 1 - par["free_surface"], 1, 1)
+    mp = oracle.oracle_abc_mpml(en, o, x, y, 20, 20, edge, *bbox, float(vp.max()), use)
+    prob = host.build_problem(tc, en, x, y, rho, c, vp, edge, 20, 20, par["free_surface"], bbox=bbox, mpml=mp)
+    rx, ry = host.station_line(par["rec_num"], par["rec_x_first"], par["rec_y_first"], par["rec_x_last"],
+                               par["rec_y_last"])
+    sx, sy = host.station_line(par["src_num"], par["src_x_first"], par["src_y_first"], par["src_x_last"],
+                               par["src_y_last"])
+    rec = np.array([host.node_location(edge, x, y, a, b) for a, b in zip(rx, ry)], np.int32)
+    src = np.array([host.node_location(edge, x, y, a, b) for a, b in zip(sx, sy)], np.int32)
+    return par, prob, src, rec
+
+
+def ex6_problem():
+    return _ex_problem(os.path.join(GOLD, "ex6"))
+
+
+def test_example6_end_to_end_character_identical():
+    par, prob, src, rec = ex6_problem()
+    # station selection reproduces receiver_station.txt / source_location.txt
+    x, y = prob["x"], prob["y"]
+    want = formats.read_text(os.path.join(GOLD, "ex6", "outputfile", "receiver_station.txt"))
+    assert "".join("%f\t%f\n" % (x[q], y[q]) for q in rec) == want
+    want = formats.read_text(os.path.join(GOLD, "ex6", "outputfile", "source_location.txt"))
+    assert "".join("%lf\t%lf\n" % (x[q], y[q]) for q in src) == want
+    r = oracle.oracle_masslump(prob, src[0], par["step"], rec, par["dt"], par["f0"], par["t0"], snap_every=200)
+    assert r["status"] == par["step"] - 2
+    out = os.path.join(GOLD, "ex6", "outputfile")
+    assert formats.format_rows(r["seis_u"]) == formats.read_text(os.path.join(out, "seismogram_u_shot_1.txt"))
+    assert formats.format_rows(r["seis_w"]) == formats.read_text(os.path.join(out, "seismogram_w_shot_1.txt"))
+    assert formats.format_rows(r["snaps_u"], formats.SNAP_SEP) == formats.read_text(
+        os.path.join(out, "wavefield_u_shot_1.txt"))
+    assert formats.format_rows(r["snaps_w"], formats.SNAP_SEP) == formats.read_text(
+        os.path.join(out, "wavefield_w_shot_1.txt"))
+    assert np.max(np.abs(r["seis_u"])) > 1.0   # a real signal, not an all-zero pin
+
+
+def test_example3_t6_seismograms_character_identical():
+    par, prob, src, rec = _ex_problem(os.path.join(GOLD, "ex3"))
+    r = oracle.oracle_masslump(prob, src[0], par["step"], rec, par["dt"], par["f0"], par["t0"])
+    out = os.path.join(GOLD, "ex3", "outputfile")
+    assert formats.format_rows(r["seis_u"]) == formats.read_text(os.path.join(out, "seismogram_u_shot_1.txt"))
+    assert formats.format_rows(r["seis_w"]) == formats.read_text(os.path.join(out, "seismogram_w_shot_1.txt"))
+
+
+def _small(nelem=12, steps=80):
+    return workloads.small_q4(nelem=nelem, steps=steps, mpml_fn=_oracle_mpml)
+
+
+def test_invariants_zero_source_and_linearity():
+    case = _small()
+    prob = case["problem"]
+    kw = dict(dt=case["dt"], f0=case["f0"], t0=case["t0"])
+    z = oracle.oracle_masslump(prob, case["src"][0], 82, case["rec"], amplitude=0.0, want_state=True, **kw)
+    assert not np.any(z["state"]) and not np.any(z["seis_u"])
+    a = oracle.oracle_masslump(prob, case["src"][0], 82, case["rec"], amplitude=1e10, **kw)
+    b = oracle.oracle_masslump(prob, case["src"][0], 82, case["rec"], amplitude=2e10, **kw)
+    scale = np.max(np.abs(a["seis_u"]))
+    assert scale > 0
+    assert np.max(np.abs(b["seis_u"] - 2.0 * a["seis_u"])) <= 1e-12 * scale
+
+
+def test_symmetry_homogeneous_centre_source():
+    """Homogeneous square, source on the symmetry line y = L/2: u is even and
+    w odd in (y - L/2) (mesh, PML and flags are mirror-symmetric in y)."""
+    nelem = 12
+    case = workloads.structured_case(
+        host.Q4, nelem, nelem, 10.0,
+        lambda d, y: workloads.isotropic_model(np.full(d.size, 2000.0), np.full(d.size, 3000.0),
+                                               np.full(d.size, 1732.0)),
+        3, 5e-4, 25.0, 0.04, 60, 20.0, 60.0, 13, 0.0, 120.0, _oracle_mpml)
+    prob = case["problem"]
+    r = oracle.oracle_masslump(prob, case["src"][0], 62, np.arange(prob["node_num"], dtype=np.int32),
+                               case["dt"], case["f0"], case["t0"])
+    x, y = prob["x"], prob["y"]
+    mirror = np.array([np.flatnonzero((x == x[i]) & (np.abs(y - (120.0 - y[i])) < 1e-9))[0]
+                       for i in range(x.size)])
+    u, w = r["seis_u"][-1], r["seis_w"][-1]
+    s = np.max(np.abs(u))
+    assert s > 0
+    assert np.max(np.abs(u - u[mirror])) < 1e-9 * s
+    # w gets the truncated-pi cos(90 deg) force, so odd symmetry holds to that size
+    assert np.max(np.abs(w + w[mirror])) < 1e-6 * max(s, np.max(np.abs(w)))
