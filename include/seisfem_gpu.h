@@ -51,8 +51,9 @@ void csr_matvec(int Bp_size, int *Bp, int *Bj, double *Bx, double *x, double *y)
 
 /* M-PML damping profiles, computed in FP64 on the GPU with the reference's
  * region tests and operand order.  Replaces pml/abc_mpml.c:1-4 (same
- * arguments).  Nodes not referenced by any element get 0 (the reference
- * leaves them unset). */
+ * arguments).  Every node gets the value of its own coordinates, which is
+ * what the reference's per-element visits give; nodes that no element
+ * references are set the same way (the reference leaves them unset). */
 void abc_mpml(int node_num, int element_num, int element_order, int *element_node,
               double **node_xy, int mpml_nx, int mpml_ny, double h, double xmin, double xmax,
               double ymin, double ymax, double vp_max, int use_mpml_xmin, int use_mpml_xmax,
@@ -62,6 +63,27 @@ void abc_mpml(int node_num, int element_num, int element_order, int *element_nod
 /* Ricker wavelet magnitude*(1-2a)exp(-a), a = (pi f0 (t-t0))^2.
  * Replaces source_receiver/seismic_source.c:1 (host function, same result). */
 double seismic_source(double f0, double t0, double magnitide, double t);
+
+/* The whole elastic_wave() with the reference's 26 arguments
+ * (time_evolution/elastic_wave.c:1-3): setup (:136-313) with the host setup of
+ * seisfem_host.h, then every shot at once on the GPU, writing
+ * ./outputfile/seismogram_{u,w}_shot_N.txt ("%f   " rows) and
+ * wavefield_{u,w}_shot_N.txt ("%f\t" rows every 200 steps) as the reference
+ * does.  Only solver "masslump" runs; other solvers and errors set
+ * sfh_driver_status() != 0 instead of exiting. */
+void elastic_wave(char *type, int node_num, int element_num, int element_order, int *element_node,
+                  double **node_xy, int nnz, int csr_p_size, int step, double dt, double f0, double t0,
+                  double edge_size, double xmin, double xmax, double ymin, double ymax, int pml_nx,
+                  int pml_ny, int src_num, int *src_node, int rec_num, int *rec_node, char *solver,
+                  int use_exterior_mesh, int free_surface_code);
+/* main() of seisfem/seisfem.c:28-299 for the run directory workdir (par.txt,
+ * mesh/, outputfile/); chdir()s into it.  Returns 0 or a negative code. */
+int seisfem_run(const char *workdir);
+/* Precision (SFG_*) and an extra full-precision binary seismogram file
+ * (outputfile/seismogram_uw_shot_N.f64) for later elastic_wave() calls. */
+void sfh_driver_options(int precision, int binary_output);
+/* Status of the last elastic_wave() call (0 = success). */
+int sfh_driver_status(void);
 
 /* ---- operators, media and runs -------------------------------------------- */
 
@@ -101,6 +123,7 @@ typedef struct {
     int shots_per_lane_group; /* SP in {1,4,16,64}, 0 = chosen from nshots */
     int guard_every;        /* energy-guard interval in steps, 0 = 50 */
     int use_graph;          /* 1 = replay captured hipGraphs of guard_every steps */
+    int xcd_remap;          /* 0 = default (on): each XCD steps one contiguous row range; -1 = off */
 } sfg_shot_desc;
 
 /* Creates a run with all state zeroed (elastic_wave.c:466-502). */
@@ -126,6 +149,20 @@ double sfg_run_last_ms_per_step(const sfg_run *run);
  * state traffic of the active shots). */
 double sfg_run_step_bytes(const sfg_run *run, int active_shots);
 void sfg_run_destroy(sfg_run *run);
+
+/* ---- time-step estimate --------------------------------------------------- */
+
+/* Largest eigenvalue of M_L^{-1} K, K the elastic operator of the loop with the
+ * M-PML terms removed (c11 S1 + c44 S2 | c13 S3 + c44 S4 on u, likewise on w)
+ * and clamped rows dropped, by FP64 power iteration on the GPU (stops when the
+ * estimate changes by <= tol relatively for 5 iterations, or at max_iter).
+ * dt_max = 2/sqrt(lambda_max) is the stability limit of the reference's
+ * Newmark update (alpha = 1, delta = 1.5 is stable for omega*dt <= 2).
+ * No counterpart in the reference: dt is a user input there (seisfem/seisfem.c:144).
+ * Returns the iteration count or an error code.  Power iteration approaches
+ * lambda_max from below, so apply a safety factor to dt_max. */
+int sfg_estimate_dt(const sfg_operator *op, const sfg_medium *md, int max_iter, double tol,
+                    double *lambda_max, double *dt_max);
 
 #ifdef __cplusplus
 }

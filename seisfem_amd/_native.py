@@ -25,6 +25,7 @@ class ShotDesc(C.Structure):
         ("max_steps", _i), ("rec_num", _i), ("rec_node", _pi),
         ("dt", _d), ("f0", _d), ("t0", _d), ("amplitude", _d), ("angle_deg", _d),
         ("block", _i), ("shots_per_lane_group", _i), ("guard_every", _i), ("use_graph", _i),
+        ("xcd_remap", _i),
     ]
 
 
@@ -77,6 +78,12 @@ def lib():
         "sfg_run_last_ms_per_step": (_d, [_vp]),
         "sfg_run_step_bytes": (_d, [_vp, _i]),
         "sfg_run_destroy": (None, [_vp]),
+        "sfg_estimate_dt": (_i, [_vp, _vp, _i, _d, _pd, _pd]),
+        "elastic_wave": (None, [C.c_char_p, _i, _i, _i, _pi, C.POINTER(_pd), _i, _i, _i, _d, _d, _d, _d,
+                                _d, _d, _d, _d, _i, _i, _i, _pi, _i, _pi, C.c_char_p, _i, _i]),
+        "seisfem_run": (_i, [C.c_char_p]),
+        "sfh_driver_options": (None, [_i, _i]),
+        "sfh_driver_status": (_i, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -548,6 +548,7 @@ extern "C" int sfg_run_create(sfg_run **out, const sfg_operator *op, const sfg_m
         a.S = r->S;
         a.rows_per_block = rows_per_block;
         a.nsteps_max = d->max_steps;
+        a.xcd_remap = d->xcd_remap >= 0 ? 1 : 0;
         a.rp = op->rp;
         a.ci = op->ci;
         for (int m = 0; m < 7; ++m) a.A[m] = vals + (size_t)m * op->nnz;
@@ -725,4 +726,56 @@ extern "C" int sfg_run_energy(sfg_run *r, double *energy, int max_checks)
             for (int k = 0; k < 2; ++k)
                 energy[((size_t)c * r->nshots + r->perm[s]) * 2 + k] = en[((size_t)c * r->S + s) * 2 + k];
     return nchk;
+}
+
+// ---------------------------------------------------------------------------
+// dt estimate (no counterpart in the reference: dt is user input, seisfem.c:144)
+
+extern "C" int sfg_estimate_dt(const sfg_operator *op, const sfg_medium *md, int max_iter, double tol,
+                               double *lambda_max, double *dt_max)
+{
+    g_err.clear();
+    if (!op || !md || md->n != op->n || max_iter <= 0 || !lambda_max || !dt_max)
+        return fail(SFG_E_ARG, "estimate_dt: bad arguments");
+    const int n = op->n;
+    const size_t n2 = 2 * (size_t)n;
+    // deterministic start vector with every component non-zero
+    std::vector<double> x0(n2);
+    uint64_t s = 0x9E3779B97F4A7C15ull;
+    double nrm = 0.0;
+    for (size_t i = 0; i < n2; ++i) {
+        s ^= s >> 12; s ^= s << 25; s ^= s >> 27;
+        x0[i] = 0.5 + (double)((s * 2685821657736338717ull) >> 11) * (1.0 / 9007199254740992.0);
+        nrm += x0[i] * x0[i];
+    }
+    for (auto &v : x0) v /= std::sqrt(nrm);
+    double *x = nullptr, *y = nullptr, *acc = nullptr;
+    int rc = dalloc(&x, n2);
+    if (!rc) rc = dalloc(&y, n2);
+    if (!rc) rc = dalloc(&acc, 1);
+    if (rc) { dfree(x); dfree(y); dfree(acc); return rc; }
+    hipError_t e = hipMemcpy(x, x0.data(), sizeof(double) * n2, hipMemcpyHostToDevice);
+    double lam = 0.0, prev = 0.0;
+    int stable = 0, it = 0;
+    for (; it < max_iter && e == hipSuccess; ++it) {
+        e = launch_fill_zero(acc, sizeof(double), nullptr);
+        if (e == hipSuccess)
+            e = launch_elastic_apply(n, op->rp, op->ci, op->v64, (size_t)op->nnz, md->coef64, md->flag, x, y, acc, nullptr);
+        double ss = 0.0;
+        if (e == hipSuccess) e = hipMemcpy(&ss, acc, sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) break;
+        lam = std::sqrt(ss); // |A x| with |x| = 1
+        if (!(lam > 0.0)) break;
+        e = launch_scale(n2, y, acc, nullptr);
+        std::swap(x, y);
+        if (std::fabs(lam - prev) <= tol * lam) { if (++stable >= 5) break; } else stable = 0;
+        prev = lam;
+    }
+    dfree(x); dfree(y); dfree(acc);
+    if (e != hipSuccess) return fail(SFG_E_HIP, "estimate_dt: %s", hipGetErrorString(e));
+    if (!(lam > 0.0)) return fail(SFG_E_ARG, "estimate_dt: operator is zero (all rows clamped?)");
+    *lambda_max = lam;
+    // the reference's Newmark variant (alpha = 1, delta = 1.5) is stable for omega*dt <= 2
+    *dt_max = 2.0 / std::sqrt(lam);
+    return it;
 }

@@ -37,6 +37,7 @@ struct StepArgs {
     int S;                  // shot stride of the state (padded shot count)
     int rows_per_block;
     int nsteps_max;
+    int xcd_remap;          // 1: contiguous block ranges per XCD
     const int *rp, *ci;     // union CSR pattern
     const T *A[7];          // M, S1..S6 on the union pattern
     const T *coef;          // NCOEF x n (x S when coefficients are per shot)
@@ -73,5 +74,9 @@ hipError_t launch_mpml(int n, const double *x, const double *y, const MpmlGeo &g
                        hipStream_t st);
 hipError_t launch_to_f32(const double *src, float *dst, size_t n, hipStream_t st);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t st);
+hipError_t launch_elastic_apply(int n, const int *rp, const int *ci, const double *A, size_t nnz,
+                                const double *coef, const int *flag, const double *x, double *y,
+                                double *sumsq, hipStream_t st);
+hipError_t launch_scale(size_t n, double *x, const double *sumsq, hipStream_t st);
 
 } // namespace sfg

@@ -41,9 +41,19 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a)
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
-    int row = (blockIdx.x * wpb + wave) * RPW + (SP >= 64 ? 0 : lane / SP);
+    // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
+    // XCD walks one contiguous range of (shot chunk, row block): neighbouring
+    // rows then share that XCD's L2.  The map is a bijection for any grid.
+    unsigned int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    if (a.xcd_remap) {
+        const unsigned int nb = gridDim.x * gridDim.y, q = nb / 8, r = nb % 8;
+        const unsigned int x = bid % 8, l = bid / 8;
+        bid = x * q + (x < r ? x : r) + l;
+    }
+    const int bx = (int)(bid % gridDim.x), by = (int)(bid / gridDim.x);
+    int row = (bx * wpb + wave) * RPW + (SP >= 64 ? 0 : lane / SP);
     if constexpr (SP >= 64) row = __builtin_amdgcn_readfirstlane(row);
-    const int s = blockIdx.y * (SP >= 64 ? 64 : SP) + (SP >= 64 ? lane : lane % SP);
+    const int s = by * (SP >= 64 ? 64 : SP) + (SP >= 64 ? lane : lane % SP);
     const int nact = (t < a.nsteps_max) ? a.active[t] : 0;
 
     if (row < a.n && s < nact) {
@@ -447,6 +457,72 @@ hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t st)
 {
     // every buffer here is a multiple of 4 bytes
     hipLaunchKernelGGL(fill_zero_kernel, dim3(2048), dim3(256), 0, st, (uint32_t *)p, bytes / 4);
+    return hipGetLastError();
+}
+
+
+// ---- dt estimate: y = M_L^{-1} K x for the elastic operator without M-PML ----------
+// K [u; w] from the d = 0 reduction of the RHS (elastic_wave.c:585-587, 651-653):
+//   (K x)_u = c11 S1 u + c44 S2 u + c13 S3 w + c44 S4 w
+//   (K x)_w = c44 S1 w + c33 S2 w + c44 S3 u + c13 S4 u
+// Clamped (Dirichlet) rows get 0.  x, y hold [u (n) ; w (n)].  sumsq[0] += |y|^2.
+__global__ void elastic_apply_kernel(int n, const int *rp, const int *ci, const double *A,
+                                     size_t nnz, const double *coef, const int *flag,
+                                     const double *x, double *y, double *sumsq)
+{
+    __shared__ double red[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (i < n) {
+        double s1u = 0, s1w = 0, s2u = 0, s2w = 0, s3u = 0, s3w = 0, s4u = 0, s4w = 0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int j = ci[k];
+            const double u = x[j], w = x[(size_t)n + j];
+            const double a1 = A[1 * nnz + k], a2 = A[2 * nnz + k], a3 = A[3 * nnz + k], a4 = A[4 * nnz + k];
+            s1u += a1 * u; s1w += a1 * w; s2u += a2 * u; s2w += a2 * w;
+            s3u += a3 * u; s3w += a3 * w; s4u += a4 * u; s4w += a4 * w;
+        }
+        const size_t N = (size_t)n;
+        const double c11 = coef[C11 * N + i], c13 = coef[C13 * N + i];
+        const double c33 = coef[C33 * N + i], c44 = coef[C44 * N + i];
+        const double m = coef[MASS * N + i];
+        double yu = 0.0, yw = 0.0;
+        if (flag[i] != 1 && m != 0.0) {
+            yu = (c11 * s1u + c44 * s2u + c13 * s3w + c44 * s4w) / m;
+            yw = (c44 * s1w + c33 * s2w + c44 * s3u + c13 * s4u) / m;
+        }
+        y[i] = yu;
+        y[N + i] = yw;
+        acc = yu * yu + yw * yw;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(sumsq, red[0]);
+}
+
+__global__ void scale_kernel(size_t n, double *x, const double *sumsq)
+{
+    const double s = sumsq[0] > 0.0 ? 1.0 / sqrt(sumsq[0]) : 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[i] *= s;
+}
+
+hipError_t launch_elastic_apply(int n, const int *rp, const int *ci, const double *A, size_t nnz,
+                                const double *coef, const int *flag, const double *x, double *y,
+                                double *sumsq, hipStream_t st)
+{
+    hipLaunchKernelGGL(elastic_apply_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, rp, ci, A, nnz,
+                       coef, flag, x, y, sumsq);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(size_t n, double *x, const double *sumsq, hipStream_t st)
+{
+    hipLaunchKernelGGL(scale_kernel, dim3(1024), dim3(256), 0, st, n, x, sumsq);
     return hipGetLastError();
 }
 

@@ -83,7 +83,7 @@ class Medium:
 class Run:
     def __init__(self, op, md, src_nodes, max_steps, rec_nodes, dt, f0, t0, amplitude=1.0e10,
                  angle_deg=90.0, precision=FP64, nsteps=None, block=0, shots_per_lane_group=0,
-                 guard_every=0, use_graph=True):
+                 guard_every=0, use_graph=True, xcd_remap=True):
         L = lib()
         self.op, self.md = op, md
         self.src = np.ascontiguousarray(np.atleast_1d(src_nodes), np.int32)
@@ -102,6 +102,7 @@ class Run:
         d.dt, d.f0, d.t0, d.amplitude, d.angle_deg = dt, f0, t0, amplitude, angle_deg
         d.block, d.shots_per_lane_group = block, shots_per_lane_group
         d.guard_every, d.use_graph = guard_every, int(bool(use_graph))
+        d.xcd_remap = 0 if xcd_remap else -1
         h = C.c_void_p()
         check(L.sfg_run_create(C.byref(h), op.h, md.h, C.byref(d)), "sfg_run_create")
         self.h = h
@@ -150,6 +151,15 @@ class Run:
             self.close()
         except Exception:
             pass
+
+
+def estimate_dt(op, md, max_iter=2000, tol=1e-9):
+    """(lambda_max, dt_max, iterations) of M_L^{-1} K by FP64 power iteration on the GPU."""
+    lam = C.c_double(0.0)
+    dtm = C.c_double(0.0)
+    it = check(lib().sfg_estimate_dt(op.h, md.h, int(max_iter), float(tol), C.byref(lam), C.byref(dtm)),
+               "sfg_estimate_dt")
+    return lam.value, dtm.value, it
 
 
 def simulate(prob, src_nodes, nsteps_exec, rec_nodes, dt, f0, t0, precision=FP64, amplitude=1.0e10,

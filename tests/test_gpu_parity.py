@@ -66,16 +66,16 @@ def _case(tc, nelem, steps, nshots=1):
     return workloads.small_q4(nelem=nelem, steps=steps, nshots=nshots, type_code=tc)
 
 
-@pytest.mark.parametrize("tc,nelem,steps,graph,guard", [
-    (host.Q4, 20, 150, True, 50), (host.Q4, 20, 150, False, 7), (host.Q9, 10, 120, True, 13),
-    (host.T6, 10, 120, True, 50), (host.T3, 20, 120, False, 50)])
-def test_fp64_loop_bit_identical(tc, nelem, steps, graph, guard):
+@pytest.mark.parametrize("tc,nelem,steps,graph,guard,xcd", [
+    (host.Q4, 20, 150, True, 50, True), (host.Q4, 20, 150, False, 7, False), (host.Q9, 10, 120, True, 13, True),
+    (host.T6, 10, 120, True, 50, True), (host.T3, 20, 120, False, 50, True)])
+def test_fp64_loop_bit_identical(tc, nelem, steps, graph, guard, xcd):
     case = _case(tc, nelem, steps)
     prob = case["problem"]
     rec = np.arange(prob["node_num"], dtype=np.int32)  # every node is a receiver
-    say(f"FP64 loop {host.TYPE_NAMES[tc]} {nelem}x{nelem} steps={steps} graph={graph} guard={guard}")
+    say(f"FP64 loop {host.TYPE_NAMES[tc]} {nelem}x{nelem} steps={steps} graph={graph} guard={guard} xcd={xcd}")
     res = gpu.simulate(prob, case["src"], steps, rec, case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
-                       use_graph=graph, guard_every=guard)
+                       use_graph=graph, guard_every=guard, xcd_remap=xcd)
     ref = oracle.oracle_masslump(prob, case["src"][0], steps + 2, rec, case["dt"], case["f0"], case["t0"])
     assert ref["status"] == steps
     assert np.max(np.abs(ref["seis_u"])) > 0
