@@ -107,6 +107,13 @@ void sfg_operator_destroy(sfg_operator *op);
 int sfg_medium_create(sfg_medium **md, const sfg_operator *op, const double *const c[4],
                       const double *mass_lump, const int *dirichlet, const double *const mpml[6]);
 void sfg_medium_destroy(sfg_medium *md);
+/* Per-row factor on the M-products of this medium (default 1).  The consistent
+ * mass samples rho at the row node (assemble/mass_sparse_q4.c:252), so a model
+ * whose density differs from the one the operator's M was assembled with uses
+ * row_scale[i] = rho_model[i] / rho_operator[i].  Applied with per-shot media
+ * (sfg_run_create_media); equal to the reference up to rounding (bit-identical
+ * only when the factor is 1). */
+int sfg_medium_set_row_scale(sfg_medium *md, const double *row_scale);
 
 typedef struct {
     int precision;          /* SFG_FP64 / SFG_FP32 / SFG_FP32_KAHAN */
@@ -129,6 +136,12 @@ typedef struct {
 /* Creates a run with all state zeroed (elastic_wave.c:466-502). */
 int sfg_run_create(sfg_run **run, const sfg_operator *op, const sfg_medium *md,
                    const sfg_shot_desc *desc);
+/* As sfg_run_create, with one medium per shot (media[nshots], all built on
+ * op): many models of one mesh share the seven operator matrices while each
+ * shot reads its own coefficients (stored [coef][node][shot] on the device).
+ * The reference runs one model per process; this batches them. */
+int sfg_run_create_media(sfg_run **run, const sfg_operator *op, const sfg_medium *const *media,
+                         const sfg_shot_desc *desc);
 /* Executes up to nsteps further steps (fewer at the end).  Returns the number
  * of steps executed so far (>= 0) or an error code. */
 int sfg_run_advance(sfg_run *run, int nsteps);

@@ -15,6 +15,7 @@ _i = C.c_int
 _d = C.c_double
 _pi = C.POINTER(C.c_int)
 _pd = C.POINTER(C.c_double)
+_pf = C.POINTER(C.c_float)
 _vp = C.c_void_p
 
 
@@ -69,7 +70,9 @@ def lib():
         "sfg_operator_destroy": (None, [_vp]),
         "sfg_medium_create": (_i, [C.POINTER(_vp), _vp, C.POINTER(_pd), _pd, _pi, C.POINTER(_pd)]),
         "sfg_medium_destroy": (None, [_vp]),
+        "sfg_medium_set_row_scale": (_i, [_vp, _pd]),
         "sfg_run_create": (_i, [C.POINTER(_vp), _vp, _vp, C.POINTER(ShotDesc)]),
+        "sfg_run_create_media": (_i, [C.POINTER(_vp), _vp, C.POINTER(_vp), C.POINTER(ShotDesc)]),
         "sfg_run_advance": (_i, [_vp, _i]),
         "sfg_run_sync": (_i, [_vp]),
         "sfg_run_seismograms": (_i, [_vp, _pd, _pd]),
@@ -84,6 +87,18 @@ def lib():
         "seisfem_run": (_i, [C.c_char_p]),
         "sfh_driver_options": (None, [_i, _i]),
         "sfh_driver_status": (_i, []),
+        # surrogate MLP (include/seisfem_surrogate.h)
+        "sfg_mlp_last_error": (C.c_char_p, []),
+        "sfg_mlp_create": (_i, [C.POINTER(_vp), _i, _i, _i, _i, _i, C.c_ulonglong]),
+        "sfg_mlp_destroy": (None, [_vp]),
+        "sfg_mlp_param_count": (C.c_longlong, [_vp]),
+        "sfg_mlp_set_data": (_i, [_vp, _pf, _pf, _i]),
+        "sfg_mlp_train_step": (_i, [_vp, _pi, C.c_float, _pd]),
+        "sfg_mlp_train": (_i, [_vp, _i, C.c_float, C.c_longlong, C.c_ulonglong, _pd, _pd]),
+        "sfg_mlp_predict": (_i, [_vp, _pf, _i, _pf, _pd]),
+        "sfg_mlp_gradients": (_i, [_vp, _pf, _pf, _pd, _i, _pf, _pf]),
+        "sfg_mlp_get_layer": (_i, [_vp, _i, _pf, _pf]),
+        "sfg_gemm_bf16_nt": (_i, [_i, _i, _i, _pf, _pf, _pf]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

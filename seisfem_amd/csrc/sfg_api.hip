@@ -307,20 +307,14 @@ extern "C" int sfg_medium_create(sfg_medium **out, const sfg_operator *op, const
     for (int k = 0; k < 6; ++k)
         if (!mpml[k]) return fail(SFG_E_ARG, "medium: null mpml[%d]", k);
     std::vector<double> h((size_t)NCOEF * n);
-    std::vector<float> h32((size_t)NCOEF * n);
     int zero = 0;
     for (int i = 0; i < n; ++i) {
         for (int k = 0; k < 4; ++k) h[(size_t)k * n + i] = c[k][i];
         for (int k = 0; k < 6; ++k) h[(size_t)(DX + k) * n + i] = mpml[k][i];
         h[(size_t)MASS * n + i] = mass_lump[i];
+        h[(size_t)MROW * n + i] = 1.0;
         if (mass_lump[i] == 0.0) ++zero;
     }
-    // FP32 copy: coefficients rounded once; 1/mass_lump formed in FP64 then
-    // rounded, with the Dirichlet clamp folded in as a zero factor.
-    for (size_t q = 0; q < (size_t)MASS * n; ++q) h32[q] = (float)h[q];
-    for (int i = 0; i < n; ++i)
-        h32[(size_t)MASS * n + i] =
-            (dirichlet[i] == 1 || mass_lump[i] == 0.0) ? 0.0f : (float)(1.0 / mass_lump[i]);
     sfg_medium *md = new sfg_medium;
     md->n = n;
     md->zero_mass = zero;
@@ -331,13 +325,26 @@ extern "C" int sfg_medium_create(sfg_medium **out, const sfg_operator *op, const
         return rc;
     }
     hipError_t e = hipMemcpy(md->coef64, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(md->coef32, h32.data(), sizeof(float) * h32.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(md->flag, dirichlet, sizeof(int) * n, hipMemcpyHostToDevice);
+    // FP32 copy on the GPU: 1/mass_lump in FP64 with the clamp folded in
+    if (e == hipSuccess) e = launch_medium_f32(n, md->coef64, md->flag, md->coef32, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         sfg_medium_destroy(md);
         return fail(SFG_E_HIP, "medium upload: %s", hipGetErrorString(e));
     }
     *out = md;
+    return SFG_OK;
+}
+
+extern "C" int sfg_medium_set_row_scale(sfg_medium *md, const double *row_scale)
+{
+    g_err.clear();
+    if (!md || !row_scale) return fail(SFG_E_ARG, "row scale: bad arguments");
+    const size_t n = (size_t)md->n;
+    HIPCHK(hipMemcpy(md->coef64 + MROW * n, row_scale, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(launch_medium_f32(md->n, md->coef64, md->flag, md->coef32, nullptr));
+    HIPCHK(hipDeviceSynchronize());
     return SFG_OK;
 }
 
@@ -359,6 +366,9 @@ struct sfg_run {
     int prec = 0, nshots = 0, S = 1, SP = 1, block = 256, nbx = 1, nby = 1;
     int max_steps = 0, rec_num = 0, guard_every = 50, nslots = 1;
     bool use_graph = true;
+    bool cps = false;           // coefficients per (node, shot): one medium per shot
+    void *coefp = nullptr;      // [NCOEF][n][S] packed coefficients (cps)
+    int *flagp = nullptr;       // [n][S] packed Dirichlet flags (cps)
     std::vector<int> perm;      // internal shot -> user shot
     std::vector<int> steps_int; // executed steps per internal shot
     void *state = nullptr, *comp = nullptr;
@@ -391,6 +401,8 @@ static void run_free(sfg_run *r)
     dfree(r->seis_u);
     dfree(r->seis_w);
     dfree(r->energy);
+    dfree(r->coefp);
+    dfree(r->flagp);
     delete r;
 }
 
@@ -404,7 +416,7 @@ extern "C" void sfg_run_destroy(sfg_run *r)
 static int launch_one_step(sfg_run *r)
 {
     const void *args = (r->prec == 0) ? (const void *)&r->a64 : (const void *)&r->a32;
-    HIPCHK(launch_step(r->prec, r->SP, false, args, r->nbx, r->nby, r->block, r->stream));
+    HIPCHK(launch_step(r->prec, r->SP, r->cps, args, r->nbx, r->nby, r->block, r->stream));
     return SFG_OK;
 }
 
@@ -416,19 +428,24 @@ static int launch_guard_now(sfg_run *r)
     return SFG_OK;
 }
 
-extern "C" int sfg_run_create(sfg_run **out, const sfg_operator *op, const sfg_medium *md,
-                              const sfg_shot_desc *d)
+static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medium *const *media, int nmedia,
+                           const sfg_shot_desc *d)
 {
     g_err.clear();
-    if (!out || !op || !md || !d) return fail(SFG_E_ARG, "run: bad arguments");
+    if (!out || !op || !media || nmedia <= 0 || !d) return fail(SFG_E_ARG, "run: bad arguments");
     *out = nullptr;
-    if (md->n != op->n) return fail(SFG_E_ARG, "run: medium and operator sizes differ");
     if (d->precision < 0 || d->precision > 2) return fail(SFG_E_ARG, "run: unknown precision %d", d->precision);
     if (d->nshots <= 0 || !d->src_node) return fail(SFG_E_ARG, "run: need >= 1 shot with a source node");
+    if (nmedia != 1 && nmedia != d->nshots) return fail(SFG_E_ARG, "run: need one medium or one per shot");
+    for (int k = 0; k < nmedia; ++k) {
+        if (!media[k] || media[k]->n != op->n) return fail(SFG_E_ARG, "run: medium %d does not match the operator", k);
+        if (d->precision != 0 && media[k]->zero_mass)
+            return fail(SFG_E_ZERO_MASS, "run: medium %d has %d zero lumped-mass entries (FP32 paths need mass > 0)",
+                        k, media[k]->zero_mass);
+    }
+    const sfg_medium *md = media[0];
     if (d->max_steps <= 0) return fail(SFG_E_ARG, "run: max_steps must be > 0");
     if (d->rec_num < 0 || (d->rec_num > 0 && !d->rec_node)) return fail(SFG_E_ARG, "run: bad receivers");
-    if (d->precision != 0 && md->zero_mass)
-        return fail(SFG_E_ZERO_MASS, "run: %d zero lumped-mass entries (FP32 paths need mass > 0)", md->zero_mass);
     const int n = op->n;
     for (int s = 0; s < d->nshots; ++s)
         if (d->src_node[s] < -1 || d->src_node[s] >= n) return fail(SFG_E_ARG, "run: source node out of range");
@@ -444,6 +461,7 @@ extern "C" int sfg_run_create(sfg_run **out, const sfg_operator *op, const sfg_m
     sfg_run *r = new sfg_run;
     r->op = op;
     r->md = md;
+    r->cps = nmedia > 1;
     r->prec = d->precision;
     r->nshots = d->nshots;
     r->max_steps = d->max_steps;
@@ -538,6 +556,34 @@ extern "C" int sfg_run_create(sfg_run **out, const sfg_operator *op, const sfg_m
         run_free(r);
         return fail(SFG_E_HIP, "run setup: %s", hipGetErrorString(e));
     }
+    if (r->cps) {
+        // one medium per shot: coefficients and flags become [.][node][shot] arrays
+        // in the internal (step-sorted) shot order
+        std::vector<const void *> csrc(d->nshots), fsrc(d->nshots);
+        for (int s = 0; s < d->nshots; ++s) {
+            const sfg_medium *m = media[r->perm[s]];
+            csrc[s] = r->prec == 0 ? (const void *)m->coef64 : (const void *)m->coef32;
+            fsrc[s] = m->flag;
+        }
+        const void **dsrc = nullptr;
+        if ((rc = dalloc((char **)&r->coefp, (size_t)NCOEF * F * w)) || (rc = dalloc(&r->flagp, F)) ||
+            (rc = dalloc((char **)&dsrc, sizeof(void *) * 2 * d->nshots))) {
+            dfree(dsrc);
+            run_free(r);
+            return rc;
+        }
+        e = hipMemcpy(dsrc, csrc.data(), sizeof(void *) * d->nshots, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(dsrc + d->nshots, fsrc.data(), sizeof(void *) * d->nshots, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = launch_pack_per_shot(dsrc, d->nshots, NCOEF, n, r->S, (int)w, r->coefp, r->stream);
+        if (e == hipSuccess) e = launch_pack_per_shot(dsrc + d->nshots, d->nshots, 1, n, r->S, 4, r->flagp, r->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+        dfree(dsrc);
+        if (e != hipSuccess) {
+            run_free(r);
+            return fail(SFG_E_HIP, "run setup (per-shot media): %s", hipGetErrorString(e));
+        }
+    }
 
     // kernel arguments (identical for every step: the step index lives on the device)
     const double pi_ref = 3.1415926535898; // elastic_wave.c:67
@@ -552,8 +598,8 @@ extern "C" int sfg_run_create(sfg_run **out, const sfg_operator *op, const sfg_m
         a.rp = op->rp;
         a.ci = op->ci;
         for (int m = 0; m < 7; ++m) a.A[m] = vals + (size_t)m * op->nnz;
-        a.coef = coef;
-        a.flag = md->flag;
+        a.coef = r->cps ? (decltype(coef))r->coefp : coef;
+        a.flag = r->cps ? r->flagp : md->flag;
         a.state = state;
         a.comp = comp;
         a.fstride = F;
@@ -603,6 +649,19 @@ extern "C" int sfg_run_create(sfg_run **out, const sfg_operator *op, const sfg_m
     }
     *out = r;
     return SFG_OK;
+}
+
+extern "C" int sfg_run_create(sfg_run **out, const sfg_operator *op, const sfg_medium *md,
+                              const sfg_shot_desc *d)
+{
+    return run_create_impl(out, op, &md, 1, d);
+}
+
+extern "C" int sfg_run_create_media(sfg_run **out, const sfg_operator *op, const sfg_medium *const *media,
+                                    const sfg_shot_desc *d)
+{
+    if (!d) return fail(SFG_E_ARG, "run: bad arguments");
+    return run_create_impl(out, op, media, d->nshots, d);
 }
 
 static int check_guard(sfg_run *r, int upto_step)
@@ -671,7 +730,9 @@ extern "C" double sfg_run_step_bytes(const sfg_run *r, int active)
     if (!r) return 0.0;
     const double N = r->op->n, nnz = r->op->nnz, w = r->prec == 0 ? 8.0 : 4.0;
     const double K = r->prec == 2 ? 2.0 * 6.0 * w : 0.0;
-    return 4.0 * (N + 1) + nnz * (4.0 + 7.0 * w) + 11.0 * N * w + (double)active * N * (52.0 * w + K);
+    // per-shot media read 12 coefficients per (node, shot) instead of 11 per node
+    const double coef = r->cps ? (double)active * N * 12.0 * w : 11.0 * N * w;
+    return 4.0 * (N + 1) + nnz * (4.0 + 7.0 * w) + coef + (double)active * N * (52.0 * w + K);
 }
 
 extern "C" int sfg_run_seismograms(sfg_run *r, double *u, double *w)

@@ -18,8 +18,12 @@ enum Field : int {
 
 // Per-node coefficient order (c11,c13,c33,c44 from model_elastic_parameter.c:7,
 // M-PML profiles from abc_mpml.c:4, then the lumped mass term).
+// MROW scales the M-products of a row: the consistent mass samples the density
+// at the row node (assemble/mass_sparse_q4.c:252), so a medium whose density
+// differs from the one the operator was assembled with is the operator's M with
+// each row multiplied by rho_medium / rho_operator.  Used with per-shot media.
 enum Coef : int {
-    C11 = 0, C13, C33, C44, DX, DY, DXX, DYY, DXXP, DYYP, MASS, NCOEF
+    C11 = 0, C13, C33, C44, DX, DY, DXX, DYY, DXXP, DYYP, MASS, MROW, NCOEF
 };
 
 // Device-side control block: the step counter lets one captured graph replay
@@ -73,7 +77,13 @@ struct MpmlGeo {
 hipError_t launch_mpml(int n, const double *x, const double *y, const MpmlGeo &g, double *out6,
                        hipStream_t st);
 hipError_t launch_to_f32(const double *src, float *dst, size_t n, hipStream_t st);
+// FP32 medium from the FP64 one: coefficients rounded once, MASS -> 1/mass_lump
+// (formed in FP64) or 0 on clamped / zero-mass nodes
+hipError_t launch_medium_f32(int n, const double *coef64, const int *flag, float *coef32, hipStream_t st);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t st);
+// dst[c][i][s] = srcs[min(s, nsrc-1)][c*n + i] for c < ncoef, i < n, s < S; elem = 4 or 8 bytes
+hipError_t launch_pack_per_shot(const void *const *srcs_dev, int nsrc, int ncoef, int n, int S, int elem,
+                                void *dst, hipStream_t st);
 hipError_t launch_elastic_apply(int n, const int *rp, const int *ci, const double *A, size_t nnz,
                                 const double *coef, const int *flag, const double *x, double *y,
                                 double *sumsq, hipStream_t st);

@@ -111,6 +111,14 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a)
         const T dxxp = a.coef[DXXP * cs + ci], dyyp = a.coef[DYYP * cs + ci];
         const T mass = a.coef[MASS * cs + ci];
         const T dt = a.dt;
+        if constexpr (CPS) {
+            // per-shot density: scale this row of the operator's M
+            const T mr = a.coef[MROW * cs + ci];
+            mu1t *= mr; mu1 *= mr; mlx1 *= mr; mlx2 *= mr; mu2t *= mr; mu2 *= mr;
+            mu3t *= mr; mu3 *= mr; mlx3 *= mr; mlx4 *= mr;
+            mw1t *= mr; mw1 *= mr; mly1 *= mr; mly2 *= mr; mw2t *= mr; mw2 *= mr;
+            mw3t *= mr; mw3 *= mr; mly3 *= mr; mly4 *= mr;
+        }
 
         // point force, (i == source_node) * point_source * sin|cos(angle)
         const double is_src = (row == a.src[s]) ? 1.0 : 0.0;
@@ -144,7 +152,7 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a)
         T an[6], ln[8];
         if constexpr (EXACT) {
             if (mass != T(0)) {
-                if (a.flag[row] == 1) {
+                if (a.flag[ci] == 1) {
 #pragma unroll
                     for (int q = 0; q < 6; ++q) an[q] = T(0);
 #pragma unroll
@@ -451,6 +459,55 @@ __global__ void fill_zero_kernel(uint32_t *p, size_t nwords)
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords;
          i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
+}
+
+__global__ void medium_f32_kernel(int n, const double *c64, const int *flag, float *c32)
+{
+    const size_t N = (size_t)n;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < (size_t)NCOEF * N;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q / N);
+        const size_t i = q % N;
+        if (k == MASS) {
+            const double m = c64[q];
+            c32[q] = (flag[i] == 1 || m == 0.0) ? 0.0f : (float)(1.0 / m);
+        } else {
+            c32[q] = (float)c64[q];
+        }
+    }
+}
+
+hipError_t launch_medium_f32(int n, const double *coef64, const int *flag, float *coef32, hipStream_t st)
+{
+    hipLaunchKernelGGL(medium_f32_kernel, dim3(1024), dim3(256), 0, st, n, coef64, flag, coef32);
+    return hipGetLastError();
+}
+
+// Per-shot coefficients: shot-innermost copy of each medium's [ncoef][n] array.
+// Padded shots (s >= nsrc) repeat the last medium so every lane reads finite data.
+template <class E>
+__global__ void pack_per_shot_kernel(const E *const *srcs, int nsrc, int ncoef, int n, int S, E *dst)
+{
+    const size_t total = (size_t)ncoef * n * S;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const int s = (int)(q % S);
+        const size_t ci = q / S; // c * n + i
+        dst[q] = srcs[s < nsrc ? s : nsrc - 1][ci];
+    }
+}
+
+hipError_t launch_pack_per_shot(const void *const *srcs_dev, int nsrc, int ncoef, int n, int S, int elem,
+                                void *dst, hipStream_t st)
+{
+    if (elem == 8)
+        hipLaunchKernelGGL(pack_per_shot_kernel<uint64_t>, dim3(2048), dim3(256), 0, st,
+                           (const uint64_t *const *)srcs_dev, nsrc, ncoef, n, S, (uint64_t *)dst);
+    else if (elem == 4)
+        hipLaunchKernelGGL(pack_per_shot_kernel<uint32_t>, dim3(2048), dim3(256), 0, st,
+                           (const uint32_t *const *)srcs_dev, nsrc, ncoef, n, S, (uint32_t *)dst);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t st)

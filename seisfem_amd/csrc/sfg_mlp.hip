@@ -1,0 +1,712 @@
+// Surrogate MLP on the GPU: bf16 MFMA GEMM (FP32 accumulate), GELU, bias, MSE,
+// Adam with FP32 master weights, and bf16 inference.  No counterpart in the
+// reference; it learns layered-model parameters -> receiver traces from shots
+// the FEM loop generates.
+//
+// Network: x[nin] -> (Linear + GELU) x nhidden_layers -> Linear -> y[nout].
+// Weights W_l [out][in] (row-major, "in" contiguous), bias b_l [out].
+// All GEMMs are C[M][N] = A[M][K] . B[N][K]^T with A and B bf16, K contiguous:
+//   forward   z = h . W^T          (A = h [batch][in],  B = W [out][in])
+//   dX        dh = dz . W          (A = dz [batch][out], B = W^T [in][out])
+//   dW        dW = dz^T . h        (A = dz^T [out][batch], B = h^T [in][batch])
+// Transposed bf16 copies are made with a transpose kernel; W^T is refreshed by
+// the Adam kernel.  Widths are padded to multiples of 32 (zeros), so every K
+// loop is whole; M / N edges are bounds-checked.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "seisfem_gpu.h"
+#include "seisfem_surrogate.h"
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    // round to nearest even (finite inputs; NaN kept quiet)
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// GELU, tanh form
+__device__ __forceinline__ float gelu(float x)
+{
+    const float k = 0.7978845608028654f; // sqrt(2/pi)
+    return 0.5f * x * (1.0f + tanhf(k * (x + 0.044715f * x * x * x)));
+}
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    const float k = 0.7978845608028654f;
+    const float u = k * (x + 0.044715f * x * x * x);
+    const float th = tanhf(u);
+    const float du = k * (1.0f + 3.0f * 0.044715f * x * x);
+    return 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * du;
+}
+
+enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2 };
+
+// ---- C = A . B^T, bf16 in, fp32 out, tile 128x128x32, 4 waves (2x2 of 64x64) ----
+constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;
+
+__global__ __launch_bounds__(256) void gemm_nt_kernel(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
+                                                      const uint16_t *__restrict__ B, int ldb, float *__restrict__ C,
+                                                      int ldc, const float *__restrict__ bias, int epi,
+                                                      uint16_t *__restrict__ Hout, int ldh)
+{
+    __shared__ uint16_t As[BM * LDK];
+    __shared__ uint16_t Bs[BN * LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    f32x16 acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+    // global -> LDS: each thread moves 16 bf16 of A and of B per K step
+    const int lr = tid >> 1, lk = (tid & 1) * 16;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        {
+            const int gm = m0 + lr;
+            uint4 v0 = make_uint4(0, 0, 0, 0), v1 = v0;
+            if (gm < M) {
+                const uint4 *p = reinterpret_cast<const uint4 *>(A + (size_t)gm * lda + k0 + lk);
+                v0 = p[0];
+                v1 = p[1];
+            }
+            uint4 *q = reinterpret_cast<uint4 *>(As + lr * LDK + lk);
+            q[0] = v0;
+            q[1] = v1;
+            const int gn = n0 + lr;
+            uint4 w0 = make_uint4(0, 0, 0, 0), w1 = w0;
+            if (gn < N) {
+                const uint4 *p = reinterpret_cast<const uint4 *>(B + (size_t)gn * ldb + k0 + lk);
+                w0 = p[0];
+                w1 = p[1];
+            }
+            uint4 *r = reinterpret_cast<uint4 *>(Bs + lr * LDK + lk);
+            r[0] = w0;
+            r[1] = w1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 8) {
+            // operand layout of v_mfma_f32_32x32x8bf16_1k: lane l holds row (l % 32),
+            // k = kk + 4 * (l / 32) + 0..3
+            const int kof = kk + 4 * (lane >> 5);
+            bf16x4 af[2], bfr[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int ra = wm * 64 + t * 32 + (lane & 31);
+                const int rb = wn * 64 + t * 32 + (lane & 31);
+                af[t] = *reinterpret_cast<const bf16x4 *>(As + ra * LDK + kof);
+                bfr[t] = *reinterpret_cast<const bf16x4 *>(Bs + rb * LDK + kof);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // epilogue: lane l, register r -> row (r/4)*8 + (l/32)*4 + r%4, column l%32
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * 64 + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                const int col = n0 + wn * 64 + b * 32 + (lane & 31);
+                if (row < M && col < N) {
+                    float v = acc[a][b][r];
+                    if (epi != EPI_STORE) v += bias[col];
+                    C[(size_t)row * ldc + col] = v;
+                    if (Hout) Hout[(size_t)row * ldh + col] = f2bf(epi == EPI_BIAS_GELU ? gelu(v) : v);
+                }
+            }
+}
+
+// dst[c][r] = src[r][c] for r < rows, c < cols (bf16)
+__global__ void transpose_bf16_kernel(int rows, int cols, const uint16_t *src, int lds, uint16_t *dst, int ldd)
+{
+    __shared__ uint16_t tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int j = threadIdx.y; j < 32; j += blockDim.y) {
+        const int r = r0 + j, c = c0 + threadIdx.x;
+        tile[j][threadIdx.x] = (r < rows && c < cols) ? src[(size_t)r * lds + c] : (uint16_t)0;
+    }
+    __syncthreads();
+    for (int j = threadIdx.y; j < 32; j += blockDim.y) {
+        const int c = c0 + j, r = r0 + threadIdx.x;
+        if (c < cols && r < rows) dst[(size_t)c * ldd + r] = tile[threadIdx.x][j];
+    }
+}
+
+__global__ void f32_to_bf16_kernel(size_t n, const float *src, uint16_t *dst)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = f2bf(src[i]);
+}
+
+// rows of X (nin wide) for the given sample indices -> bf16 [batch][ldx], zero padded
+__global__ void gather_rows_kernel(int batch, int width, int ld, const float *src, const int *idx, uint16_t *dst,
+                                   float *dstf)
+{
+    const int b = blockIdx.y;
+    if (b >= batch) return;
+    const size_t s = (size_t)idx[b];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ld; c += gridDim.x * blockDim.x) {
+        const float v = c < width ? src[s * width + c] : 0.0f;
+        if (dst) dst[(size_t)b * ld + c] = f2bf(v);
+        if (dstf) dstf[(size_t)b * ld + c] = v;
+    }
+}
+
+// dy = 2 (y - t) / (batch * nout) on valid columns; loss partial sums (y-t)^2
+__global__ void mse_grad_kernel(int batch, int nout, int ld, const float *y, const float *t, float *dy,
+                                double *loss)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    const size_t total = (size_t)batch * ld;
+    const float scale = 2.0f / ((float)batch * (float)nout);
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(q % ld);
+        float g = 0.0f;
+        if (c < nout) {
+            const float d = y[q] - t[q];
+            acc += (double)d * d;
+            g = scale * d;
+        }
+        dy[q] = g;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// dz = dh * gelu'(z) (fp32 and bf16 copies)
+__global__ void gelu_back_kernel(size_t n, const float *dh, const float *z, float *dz, uint16_t *dz16)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float g = dh[i] * gelu_grad(z[i]);
+        dz[i] = g;
+        dz16[i] = f2bf(g);
+    }
+}
+
+// db[c] = sum_b dz[b][c]
+__global__ void bias_grad_kernel(int batch, int cols, int ld, const float *dz, float *db)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.0f;
+    for (int b = 0; b < batch; ++b) s += dz[(size_t)b * ld + c];
+    db[c] = s;
+}
+
+// Adam on FP32 master weights [rows][ld] (valid cols < cols); refreshes bf16 W and W^T
+__global__ void adam_kernel(int rows, int cols, int ld, float *w, const float *g, float *m, float *v, float lr,
+                            float b1, float b2, float eps, float bc1, float bc2, uint16_t *w16, uint16_t *wt16,
+                            int ldt)
+{
+    const size_t total = (size_t)rows * ld;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(q / ld), c = (int)(q % ld);
+        if (c >= cols) continue;
+        const float gr = g[q];
+        const float mm = b1 * m[q] + (1.0f - b1) * gr;
+        const float vv = b2 * v[q] + (1.0f - b2) * gr * gr;
+        m[q] = mm;
+        v[q] = vv;
+        const float wn = w[q] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+        w[q] = wn;
+        if (w16) w16[q] = f2bf(wn);
+        if (wt16) wt16[(size_t)c * ldt + r] = f2bf(wn);
+    }
+}
+
+int pad32(int x) { return (x + 31) / 32 * 32; }
+
+thread_local std::string g_mlp_err;
+int mfail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_mlp_err = buf;
+    return code;
+}
+#define MCHK(expr)                                                                                 \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return mfail(SFG_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float *C,
+                int ldc, const float *bias, int epi, uint16_t *H, int ldh)
+{
+    if (K % BK || lda % 8 || ldb % 8) return hipErrorInvalidValue;
+    const dim3 g((N + BN - 1) / BN, (M + BM - 1) / BM);
+    hipLaunchKernelGGL(gemm_nt_kernel, g, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias, epi, H, ldh);
+    return hipGetLastError();
+}
+
+hipError_t transpose(hipStream_t st, int rows, int cols, const uint16_t *src, int lds, uint16_t *dst, int ldd)
+{
+    const dim3 g((cols + 31) / 32, (rows + 31) / 32);
+    hipLaunchKernelGGL(transpose_bf16_kernel, g, dim3(32, 8), 0, st, rows, cols, src, lds, dst, ldd);
+    return hipGetLastError();
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// the network object
+
+struct sfg_mlp {
+    int nin = 0, hid = 0, nh = 0, nout = 0, batch = 0, nl = 0;
+    std::vector<int> din, dout; // true widths per layer
+    std::vector<int> pin, pout; // padded widths
+    // per layer: master weights, grads, Adam moments (fp32 [pout][pin]), bias (+grad, moments)
+    std::vector<float *> W, G, Mw, Vw, b, gb, Mb, Vb;
+    std::vector<uint16_t *> W16, WT16;
+    // activations: h16[l] input of layer l (bf16 [batch][pin]); z[l] pre-activation fp32 [batch][pout]
+    std::vector<uint16_t *> h16, hT16;
+    std::vector<float *> z, dz, dh;
+    std::vector<uint16_t *> dz16, dzT16;
+    float *y = nullptr, *t = nullptr;
+    double *loss = nullptr;
+    int *idx = nullptr;
+    float *Xd = nullptr, *Yd = nullptr;
+    int ndata = 0;
+    long step = 0;
+    hipStream_t st = nullptr;
+};
+
+static void mlp_free(sfg_mlp *m)
+{
+    auto F = [](auto &vec) {
+        for (auto *p : vec)
+            if (p) (void)hipFree(p);
+        vec.clear();
+    };
+    F(m->W); F(m->G); F(m->Mw); F(m->Vw); F(m->b); F(m->gb); F(m->Mb); F(m->Vb);
+    F(m->W16); F(m->WT16); F(m->h16); F(m->hT16); F(m->z); F(m->dz); F(m->dh); F(m->dz16); F(m->dzT16);
+    for (void *p : {(void *)m->y, (void *)m->t, (void *)m->loss, (void *)m->idx, (void *)m->Xd, (void *)m->Yd})
+        if (p) (void)hipFree(p);
+    if (m->st) (void)hipStreamDestroy(m->st);
+    delete m;
+}
+
+template <class T>
+static hipError_t zalloc(T **p, size_t count)
+{
+    hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) e = hipMemset(*p, 0, std::max<size_t>(count, 1) * sizeof(T));
+    return e;
+}
+
+extern "C" const char *sfg_mlp_last_error(void) { return g_mlp_err.c_str(); }
+
+extern "C" int sfg_mlp_create(sfg_mlp **out, int nin, int hidden, int nhidden_layers, int nout, int batch,
+                              unsigned long long seed)
+{
+    g_mlp_err.clear();
+    if (!out || nin <= 0 || hidden <= 0 || nhidden_layers <= 0 || nout <= 0 || batch <= 0)
+        return mfail(SFG_E_ARG, "mlp: bad sizes");
+    *out = nullptr;
+    sfg_mlp *m = new sfg_mlp;
+    m->nin = nin; m->hid = hidden; m->nh = nhidden_layers; m->nout = nout; m->batch = batch;
+    m->nl = nhidden_layers + 1;
+    for (int l = 0; l < m->nl; ++l) {
+        m->din.push_back(l == 0 ? nin : hidden);
+        m->dout.push_back(l == m->nl - 1 ? nout : hidden);
+        m->pin.push_back(pad32(m->din.back()));
+        m->pout.push_back(pad32(m->dout.back()));
+    }
+    hipError_t e = hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking);
+    // He-style uniform init from a fixed-seed generator, on the host
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto rnd = [&]() {
+        s ^= s >> 12; s ^= s << 25; s ^= s >> 27;
+        return (double)((s * 2685821657736338717ull) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int l = 0; l < m->nl && e == hipSuccess; ++l) {
+        const size_t nw = (size_t)m->pout[l] * m->pin[l];
+        float *W, *G, *Mw, *Vw, *b, *gb, *Mb, *Vb;
+        uint16_t *W16, *WT16;
+        e = zalloc(&W, nw);
+        if (e == hipSuccess) e = zalloc(&G, nw);
+        if (e == hipSuccess) e = zalloc(&Mw, nw);
+        if (e == hipSuccess) e = zalloc(&Vw, nw);
+        if (e == hipSuccess) e = zalloc(&b, m->pout[l]);
+        if (e == hipSuccess) e = zalloc(&gb, m->pout[l]);
+        if (e == hipSuccess) e = zalloc(&Mb, m->pout[l]);
+        if (e == hipSuccess) e = zalloc(&Vb, m->pout[l]);
+        if (e == hipSuccess) e = zalloc(&W16, nw);
+        if (e == hipSuccess) e = zalloc(&WT16, nw);
+        m->W.push_back(W); m->G.push_back(G); m->Mw.push_back(Mw); m->Vw.push_back(Vw);
+        m->b.push_back(b); m->gb.push_back(gb); m->Mb.push_back(Mb); m->Vb.push_back(Vb);
+        m->W16.push_back(W16); m->WT16.push_back(WT16);
+        if (e != hipSuccess) break;
+        std::vector<float> hw(nw, 0.0f);
+        const double a = std::sqrt(6.0 / m->din[l]) * (l == m->nl - 1 ? 0.5 : 1.0);
+        for (int r = 0; r < m->dout[l]; ++r)
+            for (int c = 0; c < m->din[l]; ++c) hw[(size_t)r * m->pin[l] + c] = (float)((2.0 * rnd() - 1.0) * a);
+        e = hipMemcpy(W, hw.data(), sizeof(float) * nw, hipMemcpyHostToDevice);
+    }
+    for (int l = 0; l < m->nl && e == hipSuccess; ++l) {
+        uint16_t *h, *hT, *d16, *dT;
+        float *zz, *dd, *dhh;
+        e = zalloc(&h, (size_t)batch * m->pin[l]);
+        if (e == hipSuccess) e = zalloc(&hT, (size_t)batch * m->pin[l]);
+        if (e == hipSuccess) e = zalloc(&zz, (size_t)batch * m->pout[l]);
+        if (e == hipSuccess) e = zalloc(&dd, (size_t)batch * m->pout[l]);
+        if (e == hipSuccess) e = zalloc(&dhh, (size_t)batch * m->pin[l]);
+        if (e == hipSuccess) e = zalloc(&d16, (size_t)batch * m->pout[l]);
+        if (e == hipSuccess) e = zalloc(&dT, (size_t)batch * m->pout[l]);
+        m->h16.push_back(h); m->hT16.push_back(hT); m->z.push_back(zz); m->dz.push_back(dd);
+        m->dh.push_back(dhh); m->dz16.push_back(d16); m->dzT16.push_back(dT);
+    }
+    if (e == hipSuccess) e = zalloc(&m->y, (size_t)batch * m->pout.back());
+    if (e == hipSuccess) e = zalloc(&m->t, (size_t)batch * m->pout.back());
+    if (e == hipSuccess) e = zalloc(&m->loss, 1);
+    if (e == hipSuccess) e = zalloc(&m->idx, batch);
+    // bf16 weights and transposes from the master copy
+    for (int l = 0; l < m->nl && e == hipSuccess; ++l) {
+        const size_t nw = (size_t)m->pout[l] * m->pin[l];
+        hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(1024), dim3(256), 0, m->st, nw, m->W[l], m->W16[l]);
+        e = transpose(m->st, m->pout[l], m->pin[l], m->W16[l], m->pin[l], m->WT16[l], m->pout[l]);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+    if (e != hipSuccess) {
+        mlp_free(m);
+        return mfail(SFG_E_HIP, "mlp create: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return SFG_OK;
+}
+
+extern "C" void sfg_mlp_destroy(sfg_mlp *m)
+{
+    if (m) mlp_free(m);
+}
+
+extern "C" long long sfg_mlp_param_count(const sfg_mlp *m)
+{
+    long long c = 0;
+    for (int l = 0; l < m->nl; ++l) c += (long long)m->dout[l] * m->din[l] + m->dout[l];
+    return c;
+}
+
+// forward of the current batch: h16[0] holds the input; y = output (fp32)
+static int forward(sfg_mlp *m)
+{
+    for (int l = 0; l < m->nl; ++l) {
+        const bool last = l == m->nl - 1;
+        float *C = last ? m->y : m->z[l];
+        uint16_t *H = last ? nullptr : m->h16[l + 1];
+        MCHK(gemm(m->st, m->batch, m->pout[l], m->pin[l], m->h16[l], m->pin[l], m->W16[l], m->pin[l], C, m->pout[l],
+                  m->b[l], last ? EPI_BIAS : EPI_BIAS_GELU, H, last ? 0 : m->pin[l + 1]));
+    }
+    return SFG_OK;
+}
+
+// backward from dz of the last layer (already in dz[nl-1] / dz16[nl-1]) -> G, gb
+static int backward(sfg_mlp *m)
+{
+    const int B = m->batch;
+    for (int l = m->nl - 1; l >= 0; --l) {
+        // dW = dz^T . h  (A = dz^T [pout][B], B = h^T [pin][B]); K = batch (padded to 32 by construction)
+        MCHK(transpose(m->st, B, m->pout[l], m->dz16[l], m->pout[l], m->dzT16[l], B));
+        MCHK(transpose(m->st, B, m->pin[l], m->h16[l], m->pin[l], m->hT16[l], B));
+        MCHK(gemm(m->st, m->pout[l], m->pin[l], B, m->dzT16[l], B, m->hT16[l], B, m->G[l], m->pin[l], nullptr,
+                  EPI_STORE, nullptr, 0));
+        hipLaunchKernelGGL(bias_grad_kernel, dim3((m->pout[l] + 255) / 256), dim3(256), 0, m->st, B, m->dout[l],
+                           m->pout[l], m->dz[l], m->gb[l]);
+        MCHK(hipGetLastError());
+        if (l > 0) {
+            // dh = dz . W (B operand = W^T [pin][pout]) ; dz_{l-1} = dh * gelu'(z_{l-1})
+            MCHK(gemm(m->st, B, m->pin[l], m->pout[l], m->dz16[l], m->pout[l], m->WT16[l], m->pout[l], m->dh[l],
+                      m->pin[l], nullptr, EPI_STORE, nullptr, 0));
+            const size_t n = (size_t)B * m->pin[l];
+            hipLaunchKernelGGL(gelu_back_kernel, dim3(1024), dim3(256), 0, m->st, n, m->dh[l], m->z[l - 1],
+                               m->dz[l - 1], m->dz16[l - 1]);
+            MCHK(hipGetLastError());
+        }
+    }
+    return SFG_OK;
+}
+
+static int load_batch(sfg_mlp *m, const int *idx_host, const float *X, const float *Y, int nsrc)
+{
+    if (nsrc <= 0) return mfail(SFG_E_ARG, "mlp: no data");
+    for (int b = 0; b < m->batch; ++b)
+        if (idx_host[b] < 0 || idx_host[b] >= nsrc) return mfail(SFG_E_ARG, "mlp: sample index out of range");
+    MCHK(hipMemcpyAsync(m->idx, idx_host, sizeof(int) * m->batch, hipMemcpyHostToDevice, m->st));
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(1, m->batch), dim3(64), 0, m->st, m->batch, m->nin, m->pin[0], X,
+                       m->idx, m->h16[0], (float *)nullptr);
+    if (Y)
+        hipLaunchKernelGGL(gather_rows_kernel, dim3(64, m->batch), dim3(256), 0, m->st, m->batch, m->nout,
+                           m->pout.back(), Y, m->idx, (uint16_t *)nullptr, m->t);
+    MCHK(hipGetLastError());
+    return SFG_OK;
+}
+
+static int loss_grad(sfg_mlp *m, double *loss_out)
+{
+    const int l = m->nl - 1;
+    MCHK(hipMemsetAsync(m->loss, 0, sizeof(double), m->st));
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(512), dim3(256), 0, m->st, m->batch, m->nout, m->pout[l], m->y, m->t,
+                       m->dz[l], m->loss);
+    const size_t n = (size_t)m->batch * m->pout[l];
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(1024), dim3(256), 0, m->st, n, m->dz[l], m->dz16[l]);
+    MCHK(hipGetLastError());
+    if (loss_out) {
+        double h = 0.0;
+        MCHK(hipMemcpyAsync(&h, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->st));
+        MCHK(hipStreamSynchronize(m->st));
+        *loss_out = h / ((double)m->batch * m->nout);
+    }
+    return SFG_OK;
+}
+
+static int adam(sfg_mlp *m, float lr)
+{
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    m->step += 1;
+    const float bc1 = 1.0f - powf(b1, (float)m->step), bc2 = 1.0f - powf(b2, (float)m->step);
+    for (int l = 0; l < m->nl; ++l) {
+        hipLaunchKernelGGL(adam_kernel, dim3(2048), dim3(256), 0, m->st, m->dout[l], m->din[l], m->pin[l], m->W[l],
+                           m->G[l], m->Mw[l], m->Vw[l], lr, b1, b2, eps, bc1, bc2, m->W16[l], m->WT16[l], m->pout[l]);
+        hipLaunchKernelGGL(adam_kernel, dim3(8), dim3(256), 0, m->st, 1, m->dout[l], m->pout[l], m->b[l], m->gb[l],
+                           m->Mb[l], m->Vb[l], lr, b1, b2, eps, bc1, bc2, (uint16_t *)nullptr, (uint16_t *)nullptr, 0);
+        MCHK(hipGetLastError());
+    }
+    return SFG_OK;
+}
+
+extern "C" int sfg_mlp_set_data(sfg_mlp *m, const float *X, const float *Y, int n)
+{
+    g_mlp_err.clear();
+    if (!m || !X || !Y || n <= 0) return mfail(SFG_E_ARG, "mlp data: bad arguments");
+    if (m->Xd) (void)hipFree(m->Xd);
+    if (m->Yd) (void)hipFree(m->Yd);
+    m->Xd = m->Yd = nullptr;
+    MCHK(hipMalloc((void **)&m->Xd, sizeof(float) * (size_t)n * m->nin));
+    MCHK(hipMalloc((void **)&m->Yd, sizeof(float) * (size_t)n * m->nout));
+    MCHK(hipMemcpy(m->Xd, X, sizeof(float) * (size_t)n * m->nin, hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(m->Yd, Y, sizeof(float) * (size_t)n * m->nout, hipMemcpyHostToDevice));
+    m->ndata = n;
+    return SFG_OK;
+}
+
+// One Adam step on the samples idx[batch] of the resident data; returns the batch loss.
+extern "C" int sfg_mlp_train_step(sfg_mlp *m, const int *idx, float lr, double *loss)
+{
+    g_mlp_err.clear();
+    if (!m || !idx || !m->Xd) return mfail(SFG_E_ARG, "mlp train: no data");
+    int rc = load_batch(m, idx, m->Xd, m->Yd, m->ndata);
+    if (!rc) rc = forward(m);
+    if (!rc) rc = loss_grad(m, nullptr);
+    if (!rc) rc = backward(m);
+    if (!rc) rc = adam(m, lr);
+    if (!rc && loss) {
+        double h = 0.0;
+        MCHK(hipMemcpyAsync(&h, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->st));
+        MCHK(hipStreamSynchronize(m->st));
+        *loss = h / ((double)m->batch * m->nout);
+    }
+    return rc;
+}
+
+// Runs `epochs` epochs over the resident data with a host-shuffled order,
+// cosine-decayed learning rate from lr0 to 0 over total_steps (0 = this call).
+extern "C" int sfg_mlp_train(sfg_mlp *m, int epochs, float lr0, long long total_steps, unsigned long long seed,
+                             double *last_loss, double *ms_per_step)
+{
+    g_mlp_err.clear();
+    if (!m || !m->Xd || epochs <= 0) return mfail(SFG_E_ARG, "mlp train: bad arguments");
+    const int n = m->ndata, B = m->batch;
+    const int per_epoch = n / B;
+    if (per_epoch <= 0) return mfail(SFG_E_ARG, "mlp train: fewer samples than one batch");
+    const long long T = total_steps > 0 ? total_steps : (long long)per_epoch * epochs;
+    std::vector<int> order(n);
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 7;
+    hipEvent_t e0, e1;
+    MCHK(hipEventCreate(&e0));
+    MCHK(hipEventCreate(&e1));
+    MCHK(hipEventRecord(e0, m->st));
+    long long done = 0;
+    for (int ep = 0; ep < epochs; ++ep) {
+        for (int i = 0; i < n; ++i) order[i] = i;
+        for (int i = n - 1; i > 0; --i) {
+            s ^= s >> 12; s ^= s << 25; s ^= s >> 27;
+            const int j = (int)((s * 2685821657736338717ull) % (uint64_t)(i + 1));
+            std::swap(order[i], order[j]);
+        }
+        for (int k = 0; k < per_epoch; ++k) {
+            const double prog = std::min(1.0, (double)(m->step) / (double)T);
+            const float lr = (float)(lr0 * 0.5 * (1.0 + std::cos(M_PI * prog)));
+            int rc = load_batch(m, order.data() + (size_t)k * B, m->Xd, m->Yd, n);
+            if (!rc) rc = forward(m);
+            if (!rc) rc = loss_grad(m, nullptr);
+            if (!rc) rc = backward(m);
+            if (!rc) rc = adam(m, lr);
+            if (rc) return rc;
+            // the index upload reads host memory asynchronously: keep it alive
+            MCHK(hipStreamSynchronize(m->st));
+            ++done;
+        }
+    }
+    MCHK(hipEventRecord(e1, m->st));
+    MCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    MCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms_per_step) *ms_per_step = done ? ms / done : 0.0;
+    if (last_loss) {
+        double h = 0.0;
+        MCHK(hipMemcpy(&h, m->loss, sizeof(double), hipMemcpyDeviceToHost));
+        *last_loss = h / ((double)B * m->nout);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return SFG_OK;
+}
+
+// bf16 inference: Y[n][nout] = net(X[n][nin]) (host arrays), batch by batch
+extern "C" int sfg_mlp_predict(sfg_mlp *m, const float *X, int n, float *Y, double *ms_total)
+{
+    g_mlp_err.clear();
+    if (!m || !X || !Y || n <= 0) return mfail(SFG_E_ARG, "mlp predict: bad arguments");
+    const int B = m->batch, L = m->nl - 1;
+    float *Xd = nullptr;
+    MCHK(hipMalloc((void **)&Xd, sizeof(float) * (size_t)n * m->nin));
+    MCHK(hipMemcpy(Xd, X, sizeof(float) * (size_t)n * m->nin, hipMemcpyHostToDevice));
+    std::vector<int> idx(B);
+    std::vector<float> yb((size_t)B * m->pout[L]);
+    hipEvent_t e0, e1;
+    MCHK(hipEventCreate(&e0));
+    MCHK(hipEventCreate(&e1));
+    float total = 0.f;
+    for (int s0 = 0; s0 < n; s0 += B) {
+        const int cnt = std::min(B, n - s0);
+        for (int b = 0; b < B; ++b) idx[b] = s0 + std::min(b, cnt - 1);
+        int rc = load_batch(m, idx.data(), Xd, nullptr, n);
+        MCHK(hipEventRecord(e0, m->st));
+        if (!rc) rc = forward(m);
+        MCHK(hipEventRecord(e1, m->st));
+        if (rc) { (void)hipFree(Xd); return rc; }
+        MCHK(hipMemcpyAsync(yb.data(), m->y, sizeof(float) * yb.size(), hipMemcpyDeviceToHost, m->st));
+        MCHK(hipStreamSynchronize(m->st));
+        float ms = 0.f;
+        MCHK(hipEventElapsedTime(&ms, e0, e1));
+        total += ms;
+        for (int b = 0; b < cnt; ++b)
+            std::memcpy(Y + (size_t)(s0 + b) * m->nout, yb.data() + (size_t)b * m->pout[L], sizeof(float) * m->nout);
+    }
+    (void)hipFree(Xd);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_total) *ms_total = total;
+    return SFG_OK;
+}
+
+// Loss and gradients for one given batch without updating (gradient checks).
+// Xb [batch][nin], Yb [batch][nout] host; grads of layer l: dW [dout][din], db [dout]
+extern "C" int sfg_mlp_gradients(sfg_mlp *m, const float *Xb, const float *Yb, double *loss, int layer, float *dW,
+                                 float *db)
+{
+    g_mlp_err.clear();
+    if (!m || !Xb || !Yb || layer < 0 || layer >= m->nl) return mfail(SFG_E_ARG, "mlp grads: bad arguments");
+    float *Xd = nullptr, *Yd = nullptr;
+    MCHK(hipMalloc((void **)&Xd, sizeof(float) * (size_t)m->batch * m->nin));
+    MCHK(hipMalloc((void **)&Yd, sizeof(float) * (size_t)m->batch * m->nout));
+    MCHK(hipMemcpy(Xd, Xb, sizeof(float) * (size_t)m->batch * m->nin, hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(Yd, Yb, sizeof(float) * (size_t)m->batch * m->nout, hipMemcpyHostToDevice));
+    std::vector<int> idx(m->batch);
+    for (int b = 0; b < m->batch; ++b) idx[b] = b;
+    int rc = load_batch(m, idx.data(), Xd, Yd, m->batch);
+    if (!rc) rc = forward(m);
+    if (!rc) rc = loss_grad(m, loss);
+    if (!rc) rc = backward(m);
+    if (!rc) {
+        std::vector<float> g((size_t)m->pout[layer] * m->pin[layer]), gb(m->pout[layer]);
+        hipError_t e = hipStreamSynchronize(m->st);
+        if (e == hipSuccess) e = hipMemcpy(g.data(), m->G[layer], sizeof(float) * g.size(), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(gb.data(), m->gb[layer], sizeof(float) * gb.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = mfail(SFG_E_HIP, "mlp grads: %s", hipGetErrorString(e));
+        for (int r = 0; !rc && r < m->dout[layer]; ++r) {
+            if (dW) std::memcpy(dW + (size_t)r * m->din[layer], g.data() + (size_t)r * m->pin[layer],
+                                sizeof(float) * m->din[layer]);
+            if (db) db[r] = gb[r];
+        }
+    }
+    (void)hipFree(Xd);
+    (void)hipFree(Yd);
+    return rc;
+}
+
+// Master weights of layer l -> W [dout][din], b [dout] (host)
+extern "C" int sfg_mlp_get_layer(sfg_mlp *m, int layer, float *W, float *b)
+{
+    g_mlp_err.clear();
+    if (!m || layer < 0 || layer >= m->nl) return mfail(SFG_E_ARG, "mlp get: bad layer");
+    std::vector<float> g((size_t)m->pout[layer] * m->pin[layer]), gb(m->pout[layer]);
+    MCHK(hipStreamSynchronize(m->st));
+    MCHK(hipMemcpy(g.data(), m->W[layer], sizeof(float) * g.size(), hipMemcpyDeviceToHost));
+    MCHK(hipMemcpy(gb.data(), m->b[layer], sizeof(float) * gb.size(), hipMemcpyDeviceToHost));
+    for (int r = 0; r < m->dout[layer]; ++r) {
+        if (W) std::memcpy(W + (size_t)r * m->din[layer], g.data() + (size_t)r * m->pin[layer],
+                           sizeof(float) * m->din[layer]);
+        if (b) b[r] = gb[r];
+    }
+    return SFG_OK;
+}
+
+// Plain bf16 GEMM C = A . B^T on host arrays (A [M][K], B [N][K] as fp32, rounded
+// to bf16 here; K padded to 32 inside).  For kernel tests.
+extern "C" int sfg_gemm_bf16_nt(int M, int N, int K, const float *A, const float *B, float *C)
+{
+    g_mlp_err.clear();
+    if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return mfail(SFG_E_ARG, "gemm: bad arguments");
+    const int Kp = pad32(K);
+    std::vector<float> ap((size_t)M * Kp, 0.f), bp((size_t)N * Kp, 0.f);
+    for (int i = 0; i < M; ++i) std::memcpy(&ap[(size_t)i * Kp], A + (size_t)i * K, sizeof(float) * K);
+    for (int i = 0; i < N; ++i) std::memcpy(&bp[(size_t)i * Kp], B + (size_t)i * K, sizeof(float) * K);
+    float *da = nullptr, *db = nullptr, *dc = nullptr;
+    uint16_t *ha = nullptr, *hb = nullptr;
+    MCHK(hipMalloc((void **)&da, sizeof(float) * ap.size()));
+    MCHK(hipMalloc((void **)&db, sizeof(float) * bp.size()));
+    MCHK(hipMalloc((void **)&dc, sizeof(float) * (size_t)M * N));
+    MCHK(hipMalloc((void **)&ha, sizeof(uint16_t) * ap.size()));
+    MCHK(hipMalloc((void **)&hb, sizeof(uint16_t) * bp.size()));
+    MCHK(hipMemcpy(da, ap.data(), sizeof(float) * ap.size(), hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(db, bp.data(), sizeof(float) * bp.size(), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(256), dim3(256), 0, nullptr, ap.size(), da, ha);
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(256), dim3(256), 0, nullptr, bp.size(), db, hb);
+    MCHK(gemm(nullptr, M, N, Kp, ha, Kp, hb, Kp, dc, N, nullptr, EPI_STORE, nullptr, 0));
+    MCHK(hipMemcpy(C, dc, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToHost));
+    for (void *p : {(void *)da, (void *)db, (void *)dc, (void *)ha, (void *)hb}) (void)hipFree(p);
+    return SFG_OK;
+}

@@ -68,6 +68,11 @@ class Medium:
         self.h = h
         self.op = op
 
+    def set_row_scale(self, scale):
+        """rho_model / rho_operator per node (per-shot media only)."""
+        s = np.ascontiguousarray(scale, np.float64)
+        check(lib().sfg_medium_set_row_scale(self.h, _pd(s)), "sfg_medium_set_row_scale")
+
     def close(self):
         if self.h:
             lib().sfg_medium_destroy(self.h)
@@ -104,7 +109,14 @@ class Run:
         d.guard_every, d.use_graph = guard_every, int(bool(use_graph))
         d.xcd_remap = 0 if xcd_remap else -1
         h = C.c_void_p()
-        check(L.sfg_run_create(C.byref(h), op.h, md.h, C.byref(d)), "sfg_run_create")
+        if isinstance(md, (list, tuple)):
+            # one medium per shot (many models on one mesh)
+            if len(md) != self.nshots:
+                raise ValueError("need one Medium per shot")
+            arr = (C.c_void_p * self.nshots)(*[m.h.value for m in md])
+            check(L.sfg_run_create_media(C.byref(h), op.h, arr, C.byref(d)), "sfg_run_create_media")
+        else:
+            check(L.sfg_run_create(C.byref(h), op.h, md.h, C.byref(d)), "sfg_run_create")
         self.h = h
         self.precision = precision
 

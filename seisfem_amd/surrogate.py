@@ -1,0 +1,248 @@
+"""Learned surrogate of the FEM loop, with the FEM as fallback and judge.
+
+A layered isotropic model (3 interface depths, 4 P velocities; vs = vp/1.73,
+Gardner density) plus a source position map, through an MLP trained on the
+GPU by the library's own kernels (include/seisfem_surrogate.h), to
+trace-normalised vertical-displacement traces at surface receivers.  The
+training shots come from the FEM loop itself: many models share one mesh's
+operators (assembled with density 1) and run as one batch with per-shot media.
+
+A prediction is accepted when its relative L2 error against the FEM is below
+the threshold; otherwise the FEM result is used.  No counterpart in the
+reference (it has no learned component).
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import gpu, host
+from ._native import lib
+
+NIN = 13
+
+
+def _pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class MLPError(RuntimeError):
+    pass
+
+
+def _chk(rc, where):
+    if rc < 0:
+        raise MLPError(f"{where} failed ({rc}): {lib().sfg_mlp_last_error().decode(errors='replace')}")
+    return rc
+
+
+class MLP:
+    """nin -> hidden x layers (GELU) -> nout; bf16 MFMA GEMMs, FP32 master weights."""
+
+    def __init__(self, nin, hidden, layers, nout, batch=256, seed=0):
+        h = C.c_void_p()
+        _chk(lib().sfg_mlp_create(C.byref(h), nin, hidden, layers, nout, batch, seed), "sfg_mlp_create")
+        self.h, self.nin, self.nout, self.batch = h, nin, nout, batch
+        self.nlayers = layers + 1
+        self.widths = [nin] + [hidden] * layers + [nout]
+
+    @property
+    def params(self):
+        return lib().sfg_mlp_param_count(self.h)
+
+    def set_data(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        _chk(lib().sfg_mlp_set_data(self.h, _pf(X), _pf(Y), X.shape[0]), "sfg_mlp_set_data")
+
+    def train(self, epochs, lr=1e-3, total_steps=0, seed=0):
+        loss = C.c_double(0.0)
+        ms = C.c_double(0.0)
+        _chk(lib().sfg_mlp_train(self.h, epochs, lr, total_steps, seed, C.byref(loss), C.byref(ms)), "sfg_mlp_train")
+        return loss.value, ms.value
+
+    def train_step(self, idx, lr):
+        idx = np.ascontiguousarray(idx, np.int32)
+        loss = C.c_double(0.0)
+        _chk(lib().sfg_mlp_train_step(self.h, idx.ctypes.data_as(C.POINTER(C.c_int)), lr, C.byref(loss)),
+             "sfg_mlp_train_step")
+        return loss.value
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], self.nout), np.float32)
+        ms = C.c_double(0.0)
+        _chk(lib().sfg_mlp_predict(self.h, _pf(X), X.shape[0], _pf(Y), C.byref(ms)), "sfg_mlp_predict")
+        return Y, ms.value
+
+    def gradients(self, Xb, Yb, layer):
+        Xb = np.ascontiguousarray(Xb, np.float32)
+        Yb = np.ascontiguousarray(Yb, np.float32)
+        W, b = self.layer(layer)
+        gW = np.zeros_like(W)
+        gb = np.zeros_like(b)
+        loss = C.c_double(0.0)
+        _chk(lib().sfg_mlp_gradients(self.h, _pf(Xb), _pf(Yb), C.byref(loss), layer, _pf(gW), _pf(gb)),
+             "sfg_mlp_gradients")
+        return loss.value, gW, gb
+
+    def layer(self, l):
+        """FP32 master weights (W [out][in], b [out]) of layer l."""
+        W = np.zeros((self.widths[l + 1], self.widths[l]), np.float32)
+        b = np.zeros(self.widths[l + 1], np.float32)
+        _chk(lib().sfg_mlp_get_layer(self.h, l, _pf(W), _pf(b)), "sfg_mlp_get_layer")
+        return W, b
+
+    def close(self):
+        if self.h:
+            lib().sfg_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gemm_bf16(A, B):
+    """C = bf16(A) . bf16(B)^T with FP32 accumulation on the GPU."""
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    Cm = np.zeros((A.shape[0], B.shape[0]), np.float32)
+    _chk(lib().sfg_gemm_bf16_nt(A.shape[0], B.shape[0], A.shape[1], _pf(A), _pf(B), _pf(Cm)), "sfg_gemm_bf16_nt")
+    return Cm
+
+
+# ---- layered models and their FEM shots -------------------------------------------
+
+def sample_models(n, seed, size=2000.0, src_depth=20.0):
+    """n models: 3 sorted interface depths U[100, 1800] m, 4 increasing vp
+    U[1500, 4500] m/s, source x U[200, 1800] m.  Returns params [n][13]:
+    depths/size, vp/4500, vs/4500, src_x/size, src_depth/size (normalised inputs)."""
+    rng = np.random.default_rng(seed)
+    depths = np.sort(rng.uniform(100.0, 1800.0, (n, 3)), axis=1)
+    vp = np.sort(rng.uniform(1500.0, 4500.0, (n, 4)), axis=1)
+    sx = rng.uniform(200.0, 1800.0, n)
+    vs = vp / 1.73
+    X = np.concatenate([depths / size, vp / 4500.0, vs / 4500.0, (sx / size)[:, None],
+                        np.full((n, 1), src_depth / size)], axis=1).astype(np.float32)
+    return dict(depths=depths, vp=vp, vs=vs, src_x=sx, src_depth=src_depth, X=X)
+
+
+class FEMGenerator:
+    """Batched FEM shots of layered models on one structured Q4 mesh
+    (x = height, free surface at x = xmax, as the reference expects)."""
+
+    def __init__(self, nelem=200, edge=10.0, pml=20, dt=5e-4, steps=4000, f0=10.0, t0=0.12, nrec=64,
+                 decim=8, precision=gpu.FP32, mpml_fn=None):
+        self.nelem, self.edge, self.pml, self.dt, self.steps = nelem, edge, pml, dt, steps
+        self.f0, self.t0, self.decim, self.precision = f0, t0, decim, precision
+        L = edge * nelem
+        self.L = L
+        en, x, y = host.mesh_structured(host.Q4, nelem, nelem, 0.0, L, 0.0, L)
+        self.x, self.y = x, y
+        n = x.size
+        ones = np.ones(n)
+        # operator with density 1: M_ij = G_ij; a model's M is rho_i * G_ij (row scale)
+        ci, cj, vals = host.assemble_coo(host.Q4, en, x, y, ones)
+        self.csr = [host.coo2csr(n, ci, cj, vals[m]) for m in range(7)]
+        self.op = None  # device operator, created on the first run
+        p, j, v = self.csr[0]
+        rows = np.repeat(np.arange(n), np.diff(p))
+        self.G_row = np.bincount(rows, weights=v, minlength=n)          # row sums of G
+        self.G_diag = np.zeros(n)
+        on = j == rows
+        self.G_diag[rows[on]] = v[on]
+        self.flags = host.dirichlet_flags(x, y, edge, 0.0, L, 0.0, L, 1)
+        mp = (mpml_fn or host.abc_mpml)(en, 4, x, y, pml, pml, edge, 0.0, L, 0.0, L, 1.0, (1, 0, 1, 1))
+        self.mpml_unit = [np.asarray(a) for a in mp]  # profiles are linear in vp_max
+        self.rec = np.array([host.node_location(edge, x, y, L, yy) for yy in np.linspace(100.0, L - 100.0, nrec)],
+                            np.int32)
+        self.depth = L - x
+
+    def medium_arrays(self, depths, vp4):
+        layer = np.searchsorted(depths, self.depth, side="right")
+        vp = vp4[layer]
+        vs = vp / 1.73
+        rho = 310.0 * vp ** 0.25
+        c = host.elastic_constants(rho, vp, vs)
+        # HRZ lumping of M = diag(rho) G: diagonal rho_i G_ii scaled by sum(all)/sum(diag)
+        ratio = float(np.dot(rho, self.G_row) / np.dot(rho, self.G_diag))
+        mass = rho * self.G_diag * ratio
+        mpml = [a * float(vp.max()) for a in self.mpml_unit]
+        return c, mass, mpml, rho
+
+    def source_nodes(self, src_x, src_depth):
+        return np.array([host.node_location(self.edge, self.x, self.y, self.L - src_depth, sx) for sx in src_x],
+                        np.int32)
+
+    def run(self, models, idx, guard_every=0):
+        """FEM traces of models[idx]: array [len(idx)][nrec * nt] (trace-normalised
+        u at the receivers, every decim-th step) and the device seconds."""
+        if self.op is None:
+            self.op = gpu.Operator(self.x.size, self.csr)
+        media = []
+        for i in idx:
+            c, mass, mpml, rho = self.medium_arrays(models["depths"][i], models["vp"][i])
+            md = gpu.Medium(self.op, c, mass, self.flags, mpml)
+            md.set_row_scale(rho)
+            media.append(md)
+        src = self.source_nodes(models["src_x"][idx], models["src_depth"])
+        run = gpu.Run(self.op, media, src, self.steps, self.rec, self.dt, self.f0, self.t0,
+                      precision=self.precision, guard_every=guard_every)
+        run.advance(self.steps)
+        secs = run.last_ms_per_step * self.steps * 1e-3
+        u, _ = run.seismograms()
+        run.close()
+        for md in media:
+            md.close()
+        tr = u[:, self.decim - 1::self.decim, :]          # [shots][nt][nrec]
+        tr = np.transpose(tr, (0, 2, 1))                  # [shots][nrec][nt]
+        peak = np.max(np.abs(tr), axis=2, keepdims=True)
+        tr = tr / np.where(peak > 0, peak, 1.0)
+        return tr.reshape(len(idx), -1).astype(np.float32), secs
+
+    def generate(self, models, batch=128, log=None):
+        n = models["X"].shape[0]
+        out = []
+        secs = 0.0
+        for s0 in range(0, n, batch):
+            idx = np.arange(s0, min(n, s0 + batch))
+            Y, t = self.run(models, idx)
+            out.append(Y)
+            secs += t
+            if log:
+                log(f"FEM shots {s0 + len(idx)}/{n} ({secs:.1f}s device)")
+        return np.concatenate(out), secs
+
+    def close(self):
+        if self.op is not None:
+            self.op.close()
+            self.op = None
+
+
+def rel_l2_rows(A, B):
+    return np.linalg.norm(A - B, axis=1) / np.maximum(np.linalg.norm(B, axis=1), 1e-30)
+
+
+def evaluate(mlp, Xtest, Ytest, fem_seconds_per_shot, threshold=0.05):
+    """Surrogate predictions judged by the FEM: accepted below ``threshold``
+    relative L2, FEM fallback otherwise."""
+    t0 = time.time()
+    P, ms = mlp.predict(Xtest)
+    wall = time.time() - t0
+    err = rel_l2_rows(P, Ytest)
+    acc = err < threshold
+    n = Xtest.shape[0]
+    sur_rate = n / (ms * 1e-3) if ms > 0 else float("inf")
+    fem_rate = 1.0 / fem_seconds_per_shot
+    # with fallback: every model pays the surrogate, rejected ones also the FEM
+    combined = n / (ms * 1e-3 + (n - acc.sum()) * fem_seconds_per_shot)
+    return dict(median_rel_l2=float(np.median(err)), accepted=float(acc.mean()),
+                surrogate_seis_per_s=sur_rate, fem_seis_per_s=fem_rate,
+                combined_seis_per_s=combined, gain=combined / fem_rate, predict_wall_s=wall)

@@ -1,0 +1,113 @@
+"""Surrogate MLP kernels on the GPU against the MLP oracle (oracle/mlp_oracle.py).
+
+Tolerances: the GPU and the oracle round the same operands to bf16, so they
+differ only by FP32 accumulation order and tanh -- rel-L2 <= 2e-3 on
+gradients and outputs, 1e-5 on plain GEMMs of bf16 data.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+from seisfem_amd import surrogate
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print("[gpu-case]", msg, flush=True)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("M,N,K", [(70, 200, 45), (256, 1024, 1024), (128, 32000 // 8, 256), (33, 17, 13)])
+def test_gemm_bf16_matches_float64_of_bf16_operands(M, N, K):
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K)).astype(np.float32)
+    say(f"gemm bf16 {M}x{N}x{K}")
+    Cg = surrogate.gemm_bf16(A, B)
+    Cr = mo.bf16(A) @ mo.bf16(B).T
+    assert rel(Cg, Cr) < 1e-5
+
+
+def _net(batch=32, sizes=(13, 64, 64, 64, 96), seed=3):
+    return surrogate.MLP(sizes[0], sizes[1], len(sizes) - 2, sizes[-1], batch=batch, seed=seed)
+
+
+def test_gradients_match_oracle():
+    net = _net()
+    rng = np.random.default_rng(0)
+    X = rng.uniform(0, 1, (32, 13)).astype(np.float32)
+    T = rng.standard_normal((32, 96)).astype(np.float32)
+    Ws, bs = zip(*[net.layer(l) for l in range(net.nlayers)])
+    Ws = [w.astype(np.float64) for w in Ws]
+    bs = [b.astype(np.float64) for b in bs]
+    loss_ref, gW, gb = mo.loss_and_grads(Ws, bs, X, T, round_bf16=True)
+    say("mlp gradients 13-64-64-64-96 batch 32")
+    for l in range(net.nlayers):
+        loss, g, b = net.gradients(X, T, l)
+        assert abs(loss - loss_ref) <= 1e-4 * loss_ref
+        assert rel(g, gW[l]) < 2e-3, l
+        assert rel(b, gb[l]) < 2e-3, l
+    net.close()
+
+
+def test_adam_step_matches_oracle():
+    net = _net(seed=4)
+    rng = np.random.default_rng(1)
+    X = rng.uniform(0, 1, (32, 13)).astype(np.float32)
+    T = rng.standard_normal((32, 96)).astype(np.float32)
+    W0, b0 = zip(*[net.layer(l) for l in range(net.nlayers)])
+    _, gW, gb = mo.loss_and_grads([w.astype(np.float64) for w in W0], [b.astype(np.float64) for b in b0], X, T)
+    net.set_data(X, T)
+    say("mlp one Adam step")
+    net.train_step(np.arange(32), 1e-3)
+    for l in range(net.nlayers):
+        W1, b1 = net.layer(l)
+        wr, _, _ = mo.adam_step(W0[l].astype(np.float64), gW[l], 0.0, 0.0, 1, 1e-3)
+        br, _, _ = mo.adam_step(b0[l].astype(np.float64), gb[l], 0.0, 0.0, 1, 1e-3)
+        # first Adam step moves every weight by ~lr*sign(g): compare the moves
+        dw, dr = W1 - W0[l], wr - W0[l]
+        big = np.abs(gW[l]) > 1e-3 * np.max(np.abs(gW[l]))
+        assert np.mean(np.sign(dw[big]) == np.sign(dr[big])) > 0.99
+        assert rel(dw, dr) < 5e-2 and rel(b1 - b0[l], br - b0[l]) < 5e-2
+    net.close()
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(2)
+    X = rng.uniform(0, 1, (1024, 13)).astype(np.float32)
+    Wt = rng.standard_normal((13, 96))
+    T = np.tanh(X @ Wt).astype(np.float32)
+    net = _net(batch=64, seed=5)
+    net.set_data(X, T)
+    say("mlp training 1024 samples, 30 epochs")
+    l0 = np.mean((net.predict(X)[0] - T) ** 2)
+    net.train(30, lr=1e-3)
+    l1 = np.mean((net.predict(X)[0] - T) ** 2)
+    assert l1 < 0.1 * l0, (l0, l1)
+    net.close()
+
+
+def test_surrogate_pipeline_small():
+    """FEM-generated data -> GPU training -> evaluation with FEM fallback, on a
+    reduced mesh; reports what the surrogate gains over FEM alone."""
+    gen = surrogate.FEMGenerator(nelem=40, edge=50.0, pml=6, dt=2e-3, steps=400, f0=4.0, t0=0.3, nrec=16, decim=8)
+    models = surrogate.sample_models(640, seed=11, size=gen.L)
+    say("surrogate small: FEM 640 shots Q4 40x40, 400 steps")
+    Y, fem_s = gen.generate(models, batch=128)
+    assert np.all(np.isfinite(Y))
+    Xtr, Ytr, Xte, Yte = models["X"][:512], Y[:512], models["X"][512:], Y[512:]
+    net = surrogate.MLP(surrogate.NIN, 256, 3, Y.shape[1], batch=64, seed=1)
+    net.set_data(Xtr, Ytr)
+    l0 = float(np.mean((net.predict(Xtr)[0] - Ytr) ** 2))
+    loss, ms = net.train(60, lr=1e-3)
+    l1 = float(np.mean((net.predict(Xtr)[0] - Ytr) ** 2))
+    rep = surrogate.evaluate(net, Xte, Yte, fem_s / 640.0, threshold=0.05)
+    say(f"  train mse {l0:.4f} -> {l1:.4f}; eval {rep}")
+    assert l1 < 0.5 * l0
+    assert rep["surrogate_seis_per_s"] > rep["fem_seis_per_s"]
+    net.close()
+    gen.close()

@@ -133,6 +133,48 @@ def test_multishot_fp64_each_shot_bit_identical(nshots, sp):
     op.close()
 
 
+def test_per_shot_media_match_their_own_oracle_runs():
+    """Three models on one mesh in one run: model 0 is the operator's own
+    (bit-identical); models 1-2 differ in density and velocities and use the
+    row scale rho_k / rho_0 on M (FP64, rel-L2 <= 1e-12 vs their own oracle runs)."""
+    nelem, steps = 14, 120
+    models = [(2000.0, 3000.0), (2300.0, 3400.0), (1800.0, 2600.0)]
+    cases = []
+    for rho0, vp0 in models:
+        def model(depth, y, rho0=rho0, vp0=vp0):
+            n = depth.size
+            vp = vp0 + 0.1 * depth
+            return workloads.isotropic_model(rho0 + 0.05 * depth, vp, vp / 1.73)
+        cases.append(workloads.structured_case(host.Q4, nelem, nelem, 10.0, model, 3, 5e-4, 25.0, 0.04, steps,
+                                               20.0, [50.0, 70.0, 90.0], 9, 0.0, 140.0))
+    say("per-shot media: 3 models on Q4 14x14, FP64")
+    p0 = cases[0]["problem"]
+    op = gpu.Operator(p0["node_num"], p0["csr"])
+    mds = []
+    for k, cs in enumerate(cases):
+        p = cs["problem"]
+        md = gpu.Medium(op, p["c"], p["mass_lump"], p["dirichlet"], p["mpml"])
+        md.set_row_scale(p["rho"] / p0["rho"])
+        mds.append(md)
+    src = np.array([cs["src"][k] for k, cs in enumerate(cases)], np.int32)
+    run = gpu.Run(op, mds, src, steps, cases[0]["rec"], cases[0]["dt"], cases[0]["f0"], cases[0]["t0"],
+                  precision=gpu.FP64)
+    run.advance(steps)
+    u, w = run.seismograms()
+    for k, cs in enumerate(cases):
+        ref = oracle.oracle_masslump(cs["problem"], int(src[k]), steps + 2, cs["rec"], cs["dt"], cs["f0"], cs["t0"])
+        if k == 0:
+            assert bits_equal(u[0], ref["seis_u"]) and bits_equal(w[0], ref["seis_w"])
+        else:
+            a = np.concatenate([u[k].ravel(), w[k].ravel()])
+            b = np.concatenate([ref["seis_u"].ravel(), ref["seis_w"].ravel()])
+            assert rel_l2(a, b) <= 1e-12, (k, rel_l2(a, b))
+    run.close()
+    for md in mds:
+        md.close()
+    op.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(gpu.FP32, 1e-4), (gpu.FP32_KAHAN, 1e-4)])
 def test_fp32_within_tolerance_of_fp64_oracle(prec, tol):
     """FP32 seismograms within rel-L2 1e-4 of the FP64 oracle (the stated bar)."""

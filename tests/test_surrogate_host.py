@@ -1,0 +1,38 @@
+"""Host side of the surrogate pipeline (no GPU): the per-model medium built
+from the density-1 operator equals the direct assembly of that model."""
+import numpy as np
+
+import oracle
+from seisfem_amd import host, surrogate
+
+
+def test_per_model_medium_matches_direct_assembly():
+    gen = surrogate.FEMGenerator(nelem=12, edge=10.0, pml=3, steps=10, nrec=5, mpml_fn=oracle.oracle_abc_mpml)
+    models = surrogate.sample_models(3, seed=5, size=gen.L)
+    for k in range(3):
+        c, mass, mpml, rho = gen.medium_arrays(models["depths"][k] * 120.0 / 2000.0, models["vp"][k])
+        en, x, y = host.mesh_structured(host.Q4, 12, 12, 0.0, 120.0, 0.0, 120.0)
+        direct = host.lumped_mass(host.Q4, en, x, y, rho)
+        # equal up to the rounding of the two global sums (a wrong formula is O(1) off)
+        assert np.max(np.abs(mass - direct) / direct) < 1e-11
+        # M of the model = rho_i * G_ij (row scale of the density-1 operator)
+        ci, cj, vals = host.assemble_coo(host.Q4, en, x, y, rho)
+        p, j, v = host.coo2csr(x.size, ci, cj, vals[0])
+        gp, gj, gv = gen.csr[0]
+        assert np.array_equal(p, gp) and np.array_equal(j, gj)
+        rows = np.repeat(np.arange(x.size), np.diff(p))
+        assert np.max(np.abs(v - rho[rows] * gv) / np.abs(v)) < 1e-14
+        # M-PML profiles are linear in vp_max
+        ref = oracle.oracle_abc_mpml(en, 4, x, y, 3, 3, 10.0, 0.0, 120.0, 0.0, 120.0, float(rho.size and
+                                     np.max(models["vp"][k][np.searchsorted(models["depths"][k] * 120.0 / 2000.0,
+                                                                            120.0 - x, side="right")])),
+                                     (1, 0, 1, 1))
+        for a, b in zip(mpml, ref):
+            assert np.allclose(a, b, rtol=1e-13, atol=0.0)
+
+
+def test_sampled_models_are_ordered_and_normalised():
+    m = surrogate.sample_models(50, seed=1)
+    assert m["X"].shape == (50, surrogate.NIN)
+    assert np.all(np.diff(m["depths"], axis=1) >= 0) and np.all(np.diff(m["vp"], axis=1) >= 0)
+    assert np.all(m["X"] >= 0) and np.all(m["X"] <= 1.0)
