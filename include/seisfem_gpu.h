@@ -163,6 +163,25 @@ double sfg_run_last_ms_per_step(const sfg_run *run);
 double sfg_run_step_bytes(const sfg_run *run, int active_shots);
 void sfg_run_destroy(sfg_run *run);
 
+/* ---- several runs in one launch per step ---------------------------------- */
+
+/* Groups runs on different meshes / media / shot counts (e.g. many models of
+ * uneven size and step count) so that one kernel launch steps all of them;
+ * the launches of guard_every steps are captured in one hipGraph.  The runs
+ * must share precision, max_steps (use per-shot nsteps for uneven lengths),
+ * block size and guard interval, have 1 or >= 64-shot lane groups, and not
+ * have stepped yet.  While grouped, a run is advanced only through the group;
+ * its seismograms and snapshots are read as usual.  No counterpart in the
+ * reference, which runs one model per process. */
+typedef struct sfg_multi sfg_multi;
+int sfg_multi_create(sfg_multi **multi, sfg_run *const *runs, int nruns);
+/* Executes up to nsteps further steps of every run; returns the step count so far. */
+int sfg_multi_advance(sfg_multi *multi, int nsteps);
+double sfg_multi_last_ms_per_step(const sfg_multi *multi);
+/* Algorithmic bytes of the launch of step `step` (shots still active then). */
+double sfg_multi_step_bytes(const sfg_multi *multi, int step);
+void sfg_multi_destroy(sfg_multi *multi);
+
 /* ---- time-step estimate --------------------------------------------------- */
 
 /* Largest eigenvalue of M_L^{-1} K, K the elastic operator of the loop with the

@@ -380,6 +380,7 @@ struct sfg_run {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int executed = 0;
     double last_ms = 0.0;
+    bool in_multi = false;      // stepped by an sfg_multi (shares its step counter)
     StepArgs<double> a64;
     StepArgs<float> a32;
 };
@@ -420,11 +421,12 @@ static int launch_one_step(sfg_run *r)
     return SFG_OK;
 }
 
-static int launch_guard_now(sfg_run *r)
+// ctrl / stream default to the run's own (a multi-run passes its shared ones)
+static int launch_guard_now(sfg_run *r, const Ctrl *ctrl = nullptr, hipStream_t st = nullptr)
 {
     const size_t F = (size_t)r->op->n * r->S;
-    HIPCHK(launch_guard(r->prec, r->state, r->op->n, r->S, F, r->ctrl, r->energy, r->nslots,
-                        r->guard_every, r->stream));
+    HIPCHK(launch_guard(r->prec, r->state, r->op->n, r->S, F, ctrl ? ctrl : r->ctrl, r->energy, r->nslots,
+                        r->guard_every, st ? st : r->stream));
     return SFG_OK;
 }
 
@@ -487,9 +489,9 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         return fail(SFG_E_ARG, "run: node_num * padded shots must stay below 2^31");
     }
     r->block = d->block > 0 ? d->block : (sp == 1 && n < 256 * 1024 ? 64 : 256);
-    if (r->block % 64 != 0 || r->block > 1024) {
+    if (r->block % 64 != 0 || r->block > 256) {
         run_free(r);
-        return fail(SFG_E_ARG, "run: block must be a multiple of 64, <= 1024");
+        return fail(SFG_E_ARG, "run: block must be 64, 128, 192 or 256 threads (kernel launch bound)");
     }
     const int rows_per_block = (r->block / 64) * (sp >= 64 ? 1 : 64 / sp);
     r->nbx = (n + rows_per_block - 1) / rows_per_block;
@@ -595,6 +597,9 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.rows_per_block = rows_per_block;
         a.nsteps_max = d->max_steps;
         a.xcd_remap = d->xcd_remap >= 0 ? 1 : 0;
+        a.sp = r->SP;
+        a.cps = r->cps ? 1 : 0;
+        a.nbx = r->nbx;
         a.rp = op->rp;
         a.ci = op->ci;
         for (int m = 0; m < 7; ++m) a.A[m] = vals + (size_t)m * op->nnz;
@@ -664,7 +669,7 @@ extern "C" int sfg_run_create_media(sfg_run **out, const sfg_operator *op, const
     return run_create_impl(out, op, media, d->nshots, d);
 }
 
-static int check_guard(sfg_run *r, int upto_step)
+static int check_guard(sfg_run *r, int upto_step, const Ctrl *ctrl = nullptr)
 {
     const int nchk = std::min(r->nslots, (upto_step + r->guard_every - 1) / r->guard_every + 1);
     std::vector<double> en((size_t)nchk * r->S * 2);
@@ -677,7 +682,7 @@ static int check_guard(sfg_run *r, int upto_step)
                             r->perm[s], eu, ew, c, c * r->guard_every);
         }
     Ctrl h;
-    HIPCHK(hipMemcpy(&h, r->ctrl, sizeof h, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&h, ctrl ? ctrl : r->ctrl, sizeof h, hipMemcpyDeviceToHost));
     if (h.err & 1) return fail(SFG_E_ZERO_MASS, "zero lumped mass met (outputs of those nodes left unchanged)");
     if (h.step != r->executed) return fail(SFG_E_HIP, "device step counter %d != host %d", h.step, r->executed);
     return SFG_OK;
@@ -687,6 +692,7 @@ extern "C" int sfg_run_advance(sfg_run *r, int nsteps)
 {
     g_err.clear();
     if (!r || nsteps < 0) return fail(SFG_E_ARG, "advance: bad arguments");
+    if (r->in_multi) return fail(SFG_E_ARG, "advance: this run is stepped by a multi-run");
     int todo = std::min(nsteps, r->max_steps - r->executed);
     const int steps = todo;
     HIPCHK(hipEventRecord(r->ev0, r->stream));
@@ -839,4 +845,187 @@ extern "C" int sfg_estimate_dt(const sfg_operator *op, const sfg_medium *md, int
     // the reference's Newmark variant (alpha = 1, delta = 1.5) is stable for omega*dt <= 2
     *dt_max = 2.0 / std::sqrt(lam);
     return it;
+}
+
+// ---------------------------------------------------------------------------
+// multi-runs: several runs stepped by one kernel launch per step
+
+struct sfg_multi {
+    std::vector<sfg_run *> runs;
+    int prec = 0, block = 256, ntiles = 0, max_steps = 0, guard_every = 50, executed = 0;
+    Ctrl *ctrl = nullptr;       // shared step counter
+    void *args = nullptr;       // device StepArgs<T>[nruns]
+    int *tile_start = nullptr;  // device [nruns + 1]
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+};
+
+static void multi_free(sfg_multi *m)
+{
+    for (sfg_run *r : m->runs) r->in_multi = false;
+    if (m->graph) (void)hipGraphExecDestroy(m->graph);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    dfree(m->ctrl);
+    dfree(m->args);
+    dfree(m->tile_start);
+    delete m;
+}
+
+static int multi_launch_step(sfg_multi *m)
+{
+    HIPCHK(launch_multi_step(m->prec, m->args, m->tile_start, (int)m->runs.size(), m->ntiles, m->block, 1,
+                             m->stream));
+    return SFG_OK;
+}
+
+static int multi_launch_guards(sfg_multi *m)
+{
+    for (sfg_run *r : m->runs) {
+        int rc = launch_guard_now(r, m->ctrl, m->stream);
+        if (rc) return rc;
+    }
+    return SFG_OK;
+}
+
+extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns)
+{
+    g_err.clear();
+    if (!out || !runs || nruns <= 0) return fail(SFG_E_ARG, "multi: bad arguments");
+    *out = nullptr;
+    const sfg_run *r0 = runs[0];
+    for (int k = 0; k < nruns; ++k) {
+        const sfg_run *r = runs[k];
+        if (!r || r->in_multi) return fail(SFG_E_ARG, "multi: run %d is null or already in a multi-run", k);
+        if (r->executed != 0) return fail(SFG_E_ARG, "multi: run %d has already stepped", k);
+        if (r->prec != r0->prec || r->max_steps != r0->max_steps || r->block != r0->block ||
+            r->guard_every != r0->guard_every)
+            return fail(SFG_E_ARG, "multi: runs must share precision, max_steps, block and guard interval");
+        if (r->SP != 1 && r->SP != 64) return fail(SFG_E_ARG, "multi: runs need 1 or >= 64-shot lane groups");
+    }
+    sfg_multi *m = new sfg_multi;
+    m->prec = r0->prec;
+    m->block = r0->block;
+    m->max_steps = r0->max_steps;
+    m->guard_every = r0->guard_every;
+    std::vector<int> ts(nruns + 1, 0);
+    for (int k = 0; k < nruns; ++k) {
+        m->runs.push_back(runs[k]);
+        ts[k + 1] = ts[k] + runs[k]->nbx * runs[k]->nby;
+    }
+    m->ntiles = ts[nruns];
+    const size_t asz = m->prec == 0 ? sizeof(StepArgs<double>) : sizeof(StepArgs<float>);
+    int rc = SFG_OK;
+    if ((rc = dalloc(&m->ctrl, 1)) || (rc = dalloc((char **)&m->args, asz * nruns)) ||
+        (rc = dalloc(&m->tile_start, nruns + 1))) {
+        multi_free(m);
+        return rc;
+    }
+    // argument table: each run's own arguments with the shared step counter
+    std::vector<char> tab(asz * nruns);
+    for (int k = 0; k < nruns; ++k) {
+        if (m->prec == 0) {
+            StepArgs<double> a = runs[k]->a64;
+            a.ctrl = m->ctrl;
+            std::memcpy(tab.data() + asz * k, &a, asz);
+        } else {
+            StepArgs<float> a = runs[k]->a32;
+            a.ctrl = m->ctrl;
+            std::memcpy(tab.data() + asz * k, &a, asz);
+        }
+    }
+    hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&m->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&m->ev1);
+    if (e == hipSuccess) e = launch_fill_zero(m->ctrl, sizeof(Ctrl), m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(m->args, tab.data(), tab.size(), hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(m->tile_start, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) {
+        multi_free(m);
+        return fail(SFG_E_HIP, "multi setup: %s", hipGetErrorString(e));
+    }
+    // graph: guard_every launches, then every run's energy guard
+    hipGraph_t g = nullptr;
+    e = hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int k = 0; k < m->guard_every && rc == SFG_OK; ++k) rc = multi_launch_step(m);
+        if (rc == SFG_OK) rc = multi_launch_guards(m);
+        hipError_t e2 = hipStreamEndCapture(m->stream, &g);
+        if (rc == SFG_OK && e2 != hipSuccess) rc = fail(SFG_E_HIP, "multi graph capture: %s", hipGetErrorString(e2));
+    } else {
+        rc = fail(SFG_E_HIP, "multi graph capture: %s", hipGetErrorString(e));
+    }
+    if (rc == SFG_OK) {
+        e = hipGraphInstantiate(&m->graph, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = fail(SFG_E_HIP, "multi graph instantiate: %s", hipGetErrorString(e));
+    }
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != SFG_OK) {
+        multi_free(m);
+        return rc;
+    }
+    for (sfg_run *r : m->runs) r->in_multi = true;
+    *out = m;
+    return SFG_OK;
+}
+
+extern "C" int sfg_multi_advance(sfg_multi *m, int nsteps)
+{
+    g_err.clear();
+    if (!m || nsteps < 0) return fail(SFG_E_ARG, "multi advance: bad arguments");
+    int todo = std::min(nsteps, m->max_steps - m->executed);
+    const int steps = todo;
+    HIPCHK(hipEventRecord(m->ev0, m->stream));
+    while (todo > 0) {
+        if (todo >= m->guard_every && m->executed % m->guard_every == 0) {
+            HIPCHK(hipGraphLaunch(m->graph, m->stream));
+            m->executed += m->guard_every;
+            todo -= m->guard_every;
+        } else {
+            int rc = multi_launch_step(m);
+            if (rc) return rc;
+            m->executed += 1;
+            todo -= 1;
+            if (m->executed % m->guard_every == 0 || m->executed == m->max_steps) {
+                rc = multi_launch_guards(m);
+                if (rc) return rc;
+            }
+        }
+    }
+    HIPCHK(hipEventRecord(m->ev1, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_ms = steps > 0 ? (double)ms / steps : 0.0;
+    for (sfg_run *r : m->runs) {
+        r->executed = m->executed;
+        int rc = check_guard(r, m->executed, m->ctrl);
+        if (rc) return rc;
+    }
+    return m->executed;
+}
+
+extern "C" double sfg_multi_last_ms_per_step(const sfg_multi *m) { return m ? m->last_ms : 0.0; }
+
+extern "C" double sfg_multi_step_bytes(const sfg_multi *m, int step)
+{
+    if (!m) return 0.0;
+    double b = 0.0;
+    for (const sfg_run *r : m->runs) {
+        int act = 0;
+        for (int s = 0; s < r->nshots; ++s) act += r->steps_int[s] > step ? 1 : 0;
+        if (act) b += sfg_run_step_bytes(r, act);
+    }
+    return b;
+}
+
+extern "C" void sfg_multi_destroy(sfg_multi *m)
+{
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    multi_free(m);
 }

@@ -42,6 +42,7 @@ struct StepArgs {
     int rows_per_block;
     int nsteps_max;
     int xcd_remap;          // 1: contiguous block ranges per XCD
+    int sp, cps, nbx;       // lane grouping, per-shot coefficients, row blocks (multi-run dispatch)
     const int *rp, *ci;     // union CSR pattern
     const T *A[7];          // M, S1..S6 on the union pattern
     const T *coef;          // NCOEF x n (x S when coefficients are per shot)
@@ -63,6 +64,8 @@ struct StepArgs {
 // Launchers (defined in sfg_kernels.hip).  precision: 0 exact FP64, 1 FP32, 2 FP32+Kahan.
 hipError_t launch_step(int precision, int sp, bool coef_per_shot, const void *args, int nblk_x,
                        int nblk_y, int block, hipStream_t st);
+hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
+                             int block, int xcd_remap, hipStream_t st);
 hipError_t launch_guard(int precision, const void *state, int n, int S, size_t fstride,
                         const Ctrl *ctrl, double *energy, int nslots, int every, hipStream_t st);
 hipError_t launch_gather_uw(int precision, const void *state, int n, int S, size_t fstride,

@@ -31,26 +31,41 @@ __device__ __forceinline__ T mac(T acc, T a, T x)
     else return __builtin_fmaf(a, x, acc);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; this map makes each
+// XCD walk one contiguous range of logical blocks (shot chunk, row block), so
+// neighbouring rows share that XCD's L2.  A bijection on [0, nb) for any nb.
+__device__ __forceinline__ unsigned int xcd_contiguous(unsigned int bid, unsigned int nb)
+{
+    const unsigned int q = nb / 8, r = nb % 8;
+    const unsigned int x = bid % 8, l = bid / 8;
+    return x * q + (x < r ? x : r) + l;
+}
+
+// Publishes the end of one step kernel: the last block to finish advances
+// the device step counter (every block has read it before it counts itself).
+__device__ __forceinline__ void finish_step(Ctrl *ctrl, int t, unsigned int total)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned int prev = atomicAdd(&ctrl->done, 1u);
+        if (prev == total - 1) {
+            __hip_atomic_store(&ctrl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctrl->step, t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// One tile (logical block bx of rows, shot chunk by) of step t.
 template <class T, int PREC, int SP, bool CPS>
-__global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a)
+__device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, int by)
 {
     constexpr bool EXACT = (PREC == 0);
     constexpr bool KAHAN = (PREC == 2);
     constexpr int RPW = (SP >= 64) ? 1 : 64 / SP;
-    const int t = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
-    // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
-    // XCD walks one contiguous range of (shot chunk, row block): neighbouring
-    // rows then share that XCD's L2.  The map is a bijection for any grid.
-    unsigned int bid = blockIdx.y * gridDim.x + blockIdx.x;
-    if (a.xcd_remap) {
-        const unsigned int nb = gridDim.x * gridDim.y, q = nb / 8, r = nb % 8;
-        const unsigned int x = bid % 8, l = bid / 8;
-        bid = x * q + (x < r ? x : r) + l;
-    }
-    const int bx = (int)(bid % gridDim.x), by = (int)(bid / gridDim.x);
     int row = (bx * wpb + wave) * RPW + (SP >= 64 ? 0 : lane / SP);
     if constexpr (SP >= 64) row = __builtin_amdgcn_readfirstlane(row);
     const int s = by * (SP >= 64 ? 64 : SP) + (SP >= 64 ? lane : lane % SP);
@@ -218,17 +233,65 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a)
         }
     }
 
-    // ---- step completion: the last block advances the step counter ----------
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int total = gridDim.x * gridDim.y;
-        __threadfence();
-        const unsigned int prev = atomicAdd(&a.ctrl->done, 1u);
-        if (prev == total - 1) {
-            __hip_atomic_store(&a.ctrl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&a.ctrl->step, t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
+}
+
+template <class T, int PREC, int SP, bool CPS>
+__global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a)
+{
+    const int t = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int nb = gridDim.x * gridDim.y;
+    unsigned int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    if (a.xcd_remap) bid = xcd_contiguous(bid, nb);
+    step_tile<T, PREC, SP, CPS>(a, t, (int)(bid % gridDim.x), (int)(bid / gridDim.x));
+    finish_step(a.ctrl, t, nb);
+}
+
+// Several runs (own mesh, medium, shots and step counts) in one launch per
+// step.  tile_start[r] is the first logical block of run r (runs in order,
+// tile_start[nruns] = total); a run's tiles are its (row block, shot chunk)
+// grid, row blocks fastest.  Runs with one row per lane (SP = 1) and runs with
+// one row per wave (SP = 64) can be mixed; every run shares the step counter
+// of args[0].ctrl, and a run past its last step has no active shot.
+template <class T, int PREC>
+__global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__restrict__ args,
+                                                         const int *__restrict__ tile_start, int nruns,
+                                                         int xcd_remap)
+{
+    Ctrl *ctrl = args[0].ctrl;
+    const int t = __hip_atomic_load(&ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int nb = gridDim.x;
+    unsigned int bid = blockIdx.x;
+    if (xcd_remap) bid = xcd_contiguous(bid, nb);
+    int r = 0;
+    while (r + 1 < nruns && (int)bid >= tile_start[r + 1]) ++r;
+    const StepArgs<T> &a = args[r];
+    const int local = (int)bid - tile_start[r];
+    const int bx = local % a.nbx, by = local / a.nbx;
+    if (a.sp == 1) {
+        if (a.cps) step_tile<T, PREC, 1, true>(a, t, bx, by);
+        else step_tile<T, PREC, 1, false>(a, t, bx, by);
+    } else {
+        if (a.cps) step_tile<T, PREC, 64, true>(a, t, bx, by);
+        else step_tile<T, PREC, 64, false>(a, t, bx, by);
     }
+    finish_step(ctrl, t, nb);
+}
+
+hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
+                             int block, int xcd_remap, hipStream_t st)
+{
+    if (precision == 0)
+        hipLaunchKernelGGL((multi_step_kernel<double, 0>), dim3(ntiles), dim3(block), 0, st,
+                           (const StepArgs<double> *)args_dev, tile_start_dev, nruns, xcd_remap);
+    else if (precision == 1)
+        hipLaunchKernelGGL((multi_step_kernel<float, 1>), dim3(ntiles), dim3(block), 0, st,
+                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap);
+    else if (precision == 2)
+        hipLaunchKernelGGL((multi_step_kernel<float, 2>), dim3(ntiles), dim3(block), 0, st,
+                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 template <class T, int PREC>

@@ -165,6 +165,39 @@ class Run:
             pass
 
 
+class Multi:
+    """Several runs (own meshes, media, shots, step counts) stepped by one
+    kernel launch per step (sfg_multi_*)."""
+
+    def __init__(self, runs):
+        self.runs = list(runs)
+        arr = (C.c_void_p * len(self.runs))(*[r.h.value for r in self.runs])
+        h = C.c_void_p()
+        check(lib().sfg_multi_create(C.byref(h), arr, len(self.runs)), "sfg_multi_create")
+        self.h = h
+
+    def advance(self, nsteps):
+        return check(lib().sfg_multi_advance(self.h, int(nsteps)), "sfg_multi_advance")
+
+    @property
+    def last_ms_per_step(self):
+        return lib().sfg_multi_last_ms_per_step(self.h)
+
+    def step_bytes(self, step):
+        return lib().sfg_multi_step_bytes(self.h, int(step))
+
+    def close(self):
+        if self.h:
+            lib().sfg_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def estimate_dt(op, md, max_iter=2000, tol=1e-9):
     """(lambda_max, dt_max, iterations) of M_L^{-1} K by FP64 power iteration on the GPU."""
     lam = C.c_double(0.0)

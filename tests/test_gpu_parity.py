@@ -175,6 +175,41 @@ def test_per_shot_media_match_their_own_oracle_runs():
     op.close()
 
 
+def test_multi_run_one_launch_per_step_matches_separate_runs():
+    """Three runs on different meshes (single-shot and 70-shot lane groups,
+    uneven step counts) stepped by one launch per step: FP64 seismograms
+    bit-identical to the oracle run of each shot."""
+    specs = [(10, 1, 90), (13, 1, 55), (12, 70, 90)]
+    cases, runs, keep = [], [], []
+    for nelem, nshots, steps in specs:
+        case = _case(host.Q4, nelem, 90, nshots=nshots)
+        prob = case["problem"]
+        op = gpu.Operator(prob["node_num"], prob["csr"])
+        md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+        nsteps = np.full(nshots, steps, np.int32)
+        if nshots > 1:
+            nsteps = np.random.default_rng(1).integers(30, 91, nshots).astype(np.int32)
+            nsteps[0] = 90
+        run = gpu.Run(op, md, case["src"], 90, case["rec"], case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
+                      nsteps=nsteps, guard_every=25)
+        cases.append((case, nsteps))
+        runs.append(run)
+        keep += [op, md]
+    say("multi-run: Q4 10x10 (1 shot), 13x13 (1 shot, 55 steps), 12x12 (70 shots, uneven)")
+    multi = gpu.Multi(runs)
+    multi.advance(90)
+    for (case, nsteps), run in zip(cases, runs):
+        u, w = run.seismograms()
+        for s in sorted({0, len(nsteps) - 1, int(np.argmin(nsteps))}):
+            k = int(nsteps[s])
+            ref = oracle.oracle_masslump(case["problem"], case["src"][s], k + 2, case["rec"], case["dt"],
+                                         case["f0"], case["t0"])
+            assert bits_equal(u[s, :k], ref["seis_u"]) and bits_equal(w[s, :k], ref["seis_w"])
+    multi.close()
+    for r in runs:
+        r.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(gpu.FP32, 1e-4), (gpu.FP32_KAHAN, 1e-4)])
 def test_fp32_within_tolerance_of_fp64_oracle(prec, tol):
     """FP32 seismograms within rel-L2 1e-4 of the FP64 oracle (the stated bar)."""
