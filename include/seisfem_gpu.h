@@ -131,6 +131,12 @@ typedef struct {
     int guard_every;        /* energy-guard interval in steps, 0 = 50 */
     int use_graph;          /* 1 = replay captured hipGraphs of guard_every steps */
     int xcd_remap;          /* 0 = default (on): each XCD steps one contiguous row range; -1 = off */
+    /* Optional distributed sources (no counterpart in the reference, whose only
+     * source is the point force at src_node): shot s adds wavelet * weight to
+     * the u / w rows of src_count[s] nodes.  NULL src_count = point forces. */
+    const int *src_count;      /* [nshots] */
+    const int *src_nodes;      /* [sum of src_count] */
+    const double *src_weight;  /* [sum of src_count][2]: u and w weights */
 } sfg_shot_desc;
 
 /* Creates a run with all state zeroed (elastic_wave.c:466-502). */

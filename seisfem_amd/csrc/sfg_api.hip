@@ -374,6 +374,8 @@ struct sfg_run {
     void *state = nullptr, *comp = nullptr;
     Ctrl *ctrl = nullptr;
     int *active = nullptr, *src = nullptr, *rec_ptr = nullptr, *rec_idx = nullptr;
+    int *src_shot = nullptr;    // src = per-node entry pointers, then shot and weights per entry
+    double *src_wt = nullptr;
     double *wavelet = nullptr, *seis_u = nullptr, *seis_w = nullptr, *energy = nullptr;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -396,6 +398,8 @@ static void run_free(sfg_run *r)
     dfree(r->ctrl);
     dfree(r->active);
     dfree(r->src);
+    dfree(r->src_shot);
+    dfree(r->src_wt);
     dfree(r->rec_ptr);
     dfree(r->rec_idx);
     dfree(r->wavelet);
@@ -451,6 +455,16 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     const int n = op->n;
     for (int s = 0; s < d->nshots; ++s)
         if (d->src_node[s] < -1 || d->src_node[s] >= n) return fail(SFG_E_ARG, "run: source node out of range");
+    if (d->src_count) {
+        long long tot = 0;
+        for (int s = 0; s < d->nshots; ++s) {
+            if (d->src_count[s] < 0) return fail(SFG_E_ARG, "run: negative src_count");
+            tot += d->src_count[s];
+        }
+        if (tot > 0 && (!d->src_nodes || !d->src_weight)) return fail(SFG_E_ARG, "run: source entries missing");
+        for (long long q = 0; q < tot; ++q)
+            if (d->src_nodes[q] < -1 || d->src_nodes[q] >= n) return fail(SFG_E_ARG, "run: source entry node out of range");
+    }
     for (int q = 0; q < d->rec_num; ++q)
         if (d->rec_node[q] < 0 || d->rec_node[q] >= n)
             return fail(SFG_E_ARG, "run: receiver %d node %d out of range (the reference reads U[-1])", q, d->rec_node[q]);
@@ -501,11 +515,39 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     std::iota(r->perm.begin(), r->perm.end(), 0);
     std::stable_sort(r->perm.begin(), r->perm.end(), [&](int x, int y) { return steps[x] > steps[y]; });
     r->steps_int.resize(d->nshots);
-    std::vector<int> src_int(r->S, -1);
+    for (int s = 0; s < d->nshots; ++s) r->steps_int[s] = steps[r->perm[s]];
+    // sources grouped by node: (internal shot, u weight, w weight); a point force
+    // (the reference's only source) has weights sin/cos(angle) with the
+    // truncated pi of elastic_wave.c:67
+    const double pi_ref = 3.1415926535898;
+    const double su = sin(d->angle_deg * pi_ref / 180.0), sw = cos(d->angle_deg * pi_ref / 180.0);
+    std::vector<int> first(d->nshots + 1, 0);
+    for (int u = 0; u < d->nshots; ++u) first[u + 1] = first[u] + (d->src_count ? d->src_count[u] : 1);
+    struct SrcEntry { int node, shot; double wu, ww; };
+    std::vector<SrcEntry> ent;
     for (int s = 0; s < d->nshots; ++s) {
-        r->steps_int[s] = steps[r->perm[s]];
-        src_int[s] = d->src_node[r->perm[s]];
+        const int u = r->perm[s];
+        for (int q = first[u]; q < first[u + 1]; ++q) {
+            const int node = d->src_count ? d->src_nodes[q] : d->src_node[u];
+            const double wu = d->src_count ? d->src_weight[2 * q] : su;
+            const double ww = d->src_count ? d->src_weight[2 * q + 1] : sw;
+            if (node < 0) continue; // no source for this shot
+            bool merged = false;
+            for (auto &e2 : ent)
+                if (e2.node == node && e2.shot == s) { e2.wu += wu; e2.ww += ww; merged = true; }
+            if (!merged) ent.push_back({node, s, wu, ww});
+        }
     }
+    std::stable_sort(ent.begin(), ent.end(), [](const SrcEntry &x, const SrcEntry &y) { return x.node < y.node; });
+    std::vector<int> sptr(n + 1, 0), sshot(std::max<size_t>(ent.size(), 1), 0);
+    std::vector<double> swt(2 * std::max<size_t>(ent.size(), 1), 0.0);
+    for (size_t q = 0; q < ent.size(); ++q) {
+        sptr[ent[q].node + 1]++;
+        sshot[q] = ent[q].shot;
+        swt[2 * q] = ent[q].wu;
+        swt[2 * q + 1] = ent[q].ww;
+    }
+    for (int i = 0; i < n; ++i) sptr[i + 1] += sptr[i];
     std::vector<int> act(d->max_steps);
     for (int t = 0; t < d->max_steps; ++t) {
         int c = 0;
@@ -532,7 +574,8 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     hipError_t e = hipSuccess;
     if ((rc = dalloc((char **)&r->state, 2 * NFIELD * F * w)) ||
         (r->prec == 2 && (rc = dalloc((char **)&r->comp, 2 * 6 * F * w))) || (rc = dalloc(&r->ctrl, 1)) ||
-        (rc = dalloc(&r->active, d->max_steps)) || (rc = dalloc(&r->src, r->S)) ||
+        (rc = dalloc(&r->active, d->max_steps)) || (rc = dalloc(&r->src, n + 1)) ||
+        (rc = dalloc(&r->src_shot, sshot.size())) || (rc = dalloc(&r->src_wt, swt.size())) ||
         (rc = dalloc(&r->rec_ptr, n + 1)) || (rc = dalloc(&r->rec_idx, ridx.size())) ||
         (rc = dalloc(&r->wavelet, d->max_steps)) || (rc = dalloc(&r->seis_u, seis_n)) ||
         (rc = dalloc(&r->seis_w, seis_n)) || (rc = dalloc(&r->energy, (size_t)r->nslots * r->S * 2))) {
@@ -549,7 +592,9 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (e == hipSuccess) e = launch_fill_zero(r->seis_w, seis_n * 8, r->stream);
     if (e == hipSuccess) e = launch_fill_zero(r->energy, (size_t)r->nslots * r->S * 16, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->active, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, r->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(r->src, src_int.data(), sizeof(int) * r->S, hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->src, sptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->src_shot, sshot.data(), sizeof(int) * sshot.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->src_wt, swt.data(), sizeof(double) * swt.size(), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->rec_ptr, rptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->rec_idx, ridx.data(), sizeof(int) * ridx.size(), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->wavelet, wav.data(), sizeof(double) * wav.size(), hipMemcpyHostToDevice, r->stream);
@@ -588,8 +633,6 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     }
 
     // kernel arguments (identical for every step: the step index lives on the device)
-    const double pi_ref = 3.1415926535898; // elastic_wave.c:67
-    const double su = sin(d->angle_deg * pi_ref / 180.0), sw = cos(d->angle_deg * pi_ref / 180.0);
     const double alpha = 1.0, delta = 1.5; // elastic_wave.c:97
     auto fill = [&](auto &a, auto *state, auto *comp, auto *const vals, auto *coef, auto cast) {
         a.n = n;
@@ -609,7 +652,9 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.comp = comp;
         a.fstride = F;
         a.active = r->active;
-        a.src = r->src;
+        a.src_ptr = r->src;
+        a.src_shot = r->src_shot;
+        a.src_wt = r->src_wt;
         a.wavelet = r->wavelet;
         a.src_u = su;
         a.src_w = sw;

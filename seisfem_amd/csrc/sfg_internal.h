@@ -51,7 +51,10 @@ struct StepArgs {
     T *comp;                // [2][6][n][S] Kahan compensation of U1..W3 (or null)
     size_t fstride;         // n * S
     const int *active;      // active[t] = shots that execute step t (shots sorted)
-    const int *src;         // source node per shot (internal shot order)
+    // sources grouped by node: entries src_ptr[i]..src_ptr[i+1]-1 of node i hold
+    // (shot, u weight, w weight); a point force has weights sin/cos(angle)
+    const int *src_ptr, *src_shot;
+    const double *src_wt;   // [entries][2]
     const double *wavelet;  // Ricker value per step (time (t+3)*dt)
     double src_u, src_w;    // force direction factors sin/cos(angle)
     T dt, ca, alpha, cd, delta; // Newmark: ca = 0.5 - alpha, cd = 1 - delta

@@ -135,11 +135,17 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             mw3t *= mr; mw3 *= mr; mly3 *= mr; mly4 *= mr;
         }
 
-        // point force, (i == source_node) * point_source * sin|cos(angle)
-        const double is_src = (row == a.src[s]) ? 1.0 : 0.0;
+        // source term: the reference adds (i == source_node) * point_source * sin|cos(angle)
+        // to every row, so a row without a source still adds (0 * ps) * factor
         const double ps = a.wavelet[t];
-        const T fu = (T)(is_src * ps * a.src_u);
-        const T fw = (T)(is_src * ps * a.src_w);
+        double su = 0.0 * ps * a.src_u, sw = 0.0 * ps * a.src_w;
+        for (int q = a.src_ptr[row]; q < a.src_ptr[row + 1]; ++q)
+            if (a.src_shot[q] == s) {
+                su = 1.0 * ps * a.src_wt[2 * q];
+                sw = 1.0 * ps * a.src_wt[2 * q + 1];
+            }
+        const T fu = (T)su;
+        const T fw = (T)sw;
 
         // ---- RHS of the 14 split equations (operand order of :585-591, :651-657)
         const T ru1 = -c11 * s1u - T(2.0) * dx * mu1t - dx * dx * mu1 + mlx1 + mlx2 + fu;

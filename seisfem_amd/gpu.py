@@ -88,7 +88,9 @@ class Medium:
 class Run:
     def __init__(self, op, md, src_nodes, max_steps, rec_nodes, dt, f0, t0, amplitude=1.0e10,
                  angle_deg=90.0, precision=FP64, nsteps=None, block=0, shots_per_lane_group=0,
-                 guard_every=0, use_graph=True, xcd_remap=True):
+                 guard_every=0, use_graph=True, xcd_remap=True, sources=None):
+        """sources: optional per-shot list of (nodes, weights[k][2]) distributed
+        sources (u and w weights on each node) replacing the point forces."""
         L = lib()
         self.op, self.md = op, md
         self.src = np.ascontiguousarray(np.atleast_1d(src_nodes), np.int32)
@@ -108,6 +110,14 @@ class Run:
         d.block, d.shots_per_lane_group = block, shots_per_lane_group
         d.guard_every, d.use_graph = guard_every, int(bool(use_graph))
         d.xcd_remap = 0 if xcd_remap else -1
+        if sources is not None:
+            if len(sources) != self.nshots:
+                raise ValueError("need one source description per shot")
+            self._scount = np.array([len(nd) for nd, _ in sources], np.int32)
+            self._snodes = np.ascontiguousarray(np.concatenate([np.asarray(nd, np.int32) for nd, _ in sources]))
+            self._swt = np.ascontiguousarray(np.concatenate([np.asarray(wt, np.float64).reshape(-1, 2)
+                                                             for _, wt in sources]))
+            d.src_count, d.src_nodes, d.src_weight = _pi(self._scount), _pi(self._snodes), _pd(self._swt)
         h = C.c_void_p()
         if isinstance(md, (list, tuple)):
             # one medium per shot (many models on one mesh)

@@ -210,6 +210,49 @@ def test_multi_run_one_launch_per_step_matches_separate_runs():
         r.close()
 
 
+def test_distributed_sources():
+    """(1) a point force given as a one-entry distributed source is bit-identical
+    to the default path; (2) an explosive source on the symmetry line of a
+    homogeneous square gives u even and w odd about that line."""
+    nelem = 14
+    case = workloads.structured_case(
+        host.Q4, nelem, nelem, 10.0,
+        lambda d, y: workloads.isotropic_model(np.full(d.size, 2000.0), np.full(d.size, 3000.0),
+                                               np.full(d.size, 1732.0)),
+        3, 5e-4, 25.0, 0.04, 80, 20.0, 70.0, 15, 0.0, 140.0)
+    prob = case["problem"]
+    allnodes = np.arange(prob["node_num"], dtype=np.int32)
+    say("distributed sources Q4 14x14 FP64")
+    op = gpu.Operator(prob["node_num"], prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    kw = dict(precision=gpu.FP64)
+    r0 = gpu.Run(op, md, case["src"], 80, allnodes, case["dt"], case["f0"], case["t0"], **kw)
+    r0.advance(80)
+    u0, w0 = r0.seismograms()
+    pi_ref = 3.1415926535898
+    wt = np.array([[np.sin(90.0 * pi_ref / 180.0), np.cos(90.0 * pi_ref / 180.0)]])
+    r1 = gpu.Run(op, md, case["src"], 80, allnodes, case["dt"], case["f0"], case["t0"],
+                 sources=[(case["src"][:1], wt)], **kw)
+    r1.advance(80)
+    u1, w1 = r1.seismograms()
+    assert bits_equal(u0, u1) and bits_equal(w0, w1)
+    x, y = prob["x"], prob["y"]
+    nodes, wts = workloads.explosive_source(host.Q4, prob["element_node"], x, y, 140.0 - 35.0, 70.0)
+    r2 = gpu.Run(op, md, case["src"], 80, allnodes, case["dt"], case["f0"], case["t0"], sources=[(nodes, wts)],
+                 **kw)
+    r2.advance(80)
+    u2, w2 = r2.seismograms()
+    mirror = np.array([np.flatnonzero((x == x[i]) & (np.abs(y - (140.0 - y[i])) < 1e-9))[0] for i in range(x.size)])
+    su = np.max(np.abs(u2))
+    assert su > 0
+    assert np.max(np.abs(u2[0, -1] - u2[0, -1][mirror])) < 1e-9 * su
+    assert np.max(np.abs(w2[0, -1] + w2[0, -1][mirror])) < 1e-9 * max(su, np.max(np.abs(w2)))
+    for r in (r0, r1, r2):
+        r.close()
+    md.close()
+    op.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(gpu.FP32, 1e-4), (gpu.FP32_KAHAN, 1e-4)])
 def test_fp32_within_tolerance_of_fp64_oracle(prec, tol):
     """FP32 seismograms within rel-L2 1e-4 of the FP64 oracle (the stated bar)."""
