@@ -1,0 +1,72 @@
+"""Error of the FP32 forms against the FP64 path on the benchmark configurations.
+
+The FP64 GPU loop is bit-identical to the oracle (tests/test_gpu_parity.py),
+so at full size it stands in for the FP64 oracle here (the CPU oracle would
+take minutes per configuration).  For each configuration this prints one
+JSON line: rel-L2 of the seismograms (u and w concatenated) for FP32 and
+FP32 + Kahan, and the DOF-steps/s of each form.
+
+python -m seisfem_amd.accuracy [--configs 1,2,3] [--steps N]
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import gpu, workloads
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run_case(name, case, forms=("fp64", "fp32", "fp32_kahan"), sources=None):
+    prob = case["problem"]
+    N = prob["node_num"]
+    steps = case["steps_exec"]
+    op = gpu.Operator(N, prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    out = {"config": name, "nodes": N, "dof": 2 * N, "steps": steps, "receivers": int(case["rec"].size)}
+    ref = None
+    for f in forms:
+        t0 = time.time()
+        run = gpu.Run(op, md, case["src"][:1], steps, case["rec"], case["dt"], case["f0"], case["t0"],
+                      precision=gpu.PRECISION[f], sources=sources)
+        run.advance(steps)
+        u, w = run.seismograms()
+        ms = run.last_ms_per_step
+        run.close()
+        s = np.concatenate([u[0].ravel(), w[0].ravel()])
+        out[f] = {"ms_per_step": ms, "dof_steps_per_s": 2.0 * N / (ms * 1e-3), "wall_s": time.time() - t0}
+        if f == "fp64":
+            ref = s
+            out[f]["max_abs_u"] = float(np.max(np.abs(u)))
+        else:
+            out[f]["rel_l2_vs_fp64"] = rel_l2(s, ref)
+        print(f"[accuracy] {name} {f}: {ms:.3f} ms/step", file=sys.stderr, flush=True)
+    md.close()
+    op.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--configs", default="1,2,3")
+    ap.add_argument("--steps", type=int, default=0, help="override executed steps (0 = the configuration's)")
+    args = ap.parse_args()
+    for c in args.configs.split(","):
+        if c == "1":
+            case = workloads.config1(steps_exec=args.steps or 1998)
+            print(json.dumps(run_case("config1 Q4 200x200 homogeneous", case)), flush=True)
+        elif c == "2":
+            case = workloads.config2(steps_exec=args.steps or 4000)
+            print(json.dumps(run_case("config2 Q9 400x400 6-layer VTI", case)), flush=True)
+        elif c == "3":
+            case = workloads.config3(steps_exec=args.steps or 5000)
+            print(json.dumps(run_case("config3 T6 topography explosive", case, sources=case["sources"])), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -226,6 +226,54 @@ class FEMGenerator:
             self.op = None
 
 
+def main():
+    """Config-5 pipeline: FEM shots -> GPU training -> judged evaluation.
+
+    python -m seisfem_amd.surrogate [--train 20000] [--test 2000] [--epochs 40]
+    Prints one JSON line with the training and evaluation figures."""
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=20000)
+    ap.add_argument("--test", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--nelem", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--hidden", type=int, default=1024)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--threshold", type=float, default=0.05)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+
+    def log(m):
+        print("[surrogate]", m, file=sys.stderr, flush=True)
+
+    t0 = time.time()
+    gen = FEMGenerator(nelem=args.nelem, edge=2000.0 / args.nelem, steps=args.steps, nrec=64,
+                       decim=max(1, args.steps // 500))
+    n = args.train + args.test
+    models = sample_models(n, seed=args.seed, size=gen.L)
+    Y, fem_s = gen.generate(models, batch=128, log=log)
+    log(f"FEM data {Y.shape} in {time.time() - t0:.0f}s wall, {fem_s:.1f}s device")
+    X = models["X"]
+    net = MLP(NIN, args.hidden, 3, Y.shape[1], batch=args.batch, seed=args.seed)
+    net.set_data(X[:args.train], Y[:args.train])
+    steps_total = (args.train // args.batch) * args.epochs
+    t1 = time.time()
+    loss, ms = net.train(args.epochs, lr=1e-3, total_steps=steps_total, seed=args.seed)
+    log(f"trained {args.epochs} epochs ({steps_total} steps) in {time.time() - t1:.1f}s, {ms:.3f} ms/step, "
+        f"loss {loss:.4g}")
+    rep = evaluate(net, X[args.train:], Y[args.train:], fem_s / n, threshold=args.threshold)
+    out = {"config": f"surrogate 13-{args.hidden}x3-{Y.shape[1]}, {args.train} FEM shots Q4 {args.nelem}x"
+                     f"{args.nelem}, {args.steps} steps", "params": net.params, "train_ms_per_step": ms,
+           "final_train_mse": loss, "fem_device_s": fem_s, **rep}
+    print(json.dumps(out), flush=True)
+    net.close()
+    gen.close()
+
+
 def rel_l2_rows(A, B):
     return np.linalg.norm(A - B, axis=1) / np.maximum(np.linalg.norm(B, axis=1), 1e-30)
 
@@ -246,3 +294,7 @@ def evaluate(mlp, Xtest, Ytest, fem_seconds_per_shot, threshold=0.05):
     return dict(median_rel_l2=float(np.median(err)), accepted=float(acc.mean()),
                 surrogate_seis_per_s=sur_rate, fem_seis_per_s=fem_rate,
                 combined_seis_per_s=combined, gain=combined / fem_rate, predict_wall_s=wall)
+
+
+if __name__ == "__main__":
+    main()
