@@ -62,7 +62,7 @@ def main():
         args.nelem, args.steps, args.fp64_steps, args.cpu_steps = 100, 200, 100, 10
         args.acc_shots = 1
 
-    from seisfem_amd import gpu, workloads
+    from seisfem_amd import autotune, gpu, workloads
 
     t_setup = time.time()
     case = workloads.config4a(nshots=args.shots, nelem=args.nelem,
@@ -76,7 +76,16 @@ def main():
     prec = gpu.PRECISION[args.precision]
     op = gpu.Operator(N, prob["csr"])
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
-    kw = dict(dt=case["dt"], f0=case["f0"], t0=case["t0"], block=args.block)
+    base = dict(dt=case["dt"], f0=case["f0"], t0=case["t0"])
+    # launch shape chosen by measurement on this problem (untimed)
+    if args.block:
+        best = dict(block=args.block)
+    else:
+        best, _ = autotune.tune(op, md, case["src"], case["rec"], precision=prec, steps=30, log=log, **base)
+    best64, _ = autotune.tune(op, md, case["src"][:1], case["rec"], precision=gpu.FP64, steps=30, log=log, **base)
+    log(f"tuned batched {best}; tuned fp64 single {best64}")
+    kw = dict(base, **best)
+    kw64 = dict(base, **best64)
 
     # warmup: a separate run of the same shape (untimed)
     warm = gpu.Run(op, md, case["src"], max(args.warmup, 1), case["rec"], precision=prec, **kw)
@@ -99,7 +108,7 @@ def main():
 
     # roofline pass: all shots active, guard only at the end -> step-kernel time
     Kr = min(400, K)
-    rr = gpu.Run(op, md, case["src"], Kr, case["rec"], precision=prec, guard_every=Kr, **kw)
+    rr = gpu.Run(op, md, case["src"], Kr, case["rec"], precision=prec, **dict(kw, guard_every=Kr))
     rr.advance(Kr)
     ms_kernel = rr.last_ms_per_step
     bytes_launch = rr.step_bytes(args.shots)
@@ -108,7 +117,7 @@ def main():
     log(f"step kernel: {ms_kernel:.4f} ms, {bytes_launch / 1e9:.3f} GB algorithmic -> {achieved:.0f} GB/s")
 
     # plain single-shot FP64 loop on the same mesh (the reference-exact path)
-    r64 = gpu.Run(op, md, case["src"][:1], args.fp64_steps, case["rec"], precision=gpu.FP64, **kw)
+    r64 = gpu.Run(op, md, case["src"][:1], args.fp64_steps, case["rec"], precision=gpu.FP64, **kw64)
     r64.advance(args.fp64_steps)
     ms64 = r64.last_ms_per_step
     r64.close()
@@ -119,7 +128,7 @@ def main():
     acc = None
     if not args.no_acc:
         idx = np.linspace(0, args.shots - 1, args.acc_shots).astype(int)
-        ra = gpu.Run(op, md, case["src"][idx], K, case["rec"], precision=gpu.FP64, nsteps=nsteps[idx], **kw)
+        ra = gpu.Run(op, md, case["src"][idx], K, case["rec"], precision=gpu.FP64, nsteps=nsteps[idx], **base)
         ra.advance(K)
         u64, w64 = ra.seismograms()
         ra.close()
@@ -162,7 +171,7 @@ def main():
         "config": {"workload": f"{args.shots} shots x Q4 {args.nelem}x{args.nelem} (dx=10 m), "
                                f"uneven steps U[{int(nsteps.min())},{K}], {args.precision}",
                    "nodes": N, "dof_per_shot": 2 * N, "shots": args.shots, "precision": args.precision,
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU", "tuned": best, "tuned_fp64": best64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "step_kernel (fused SpMM + M-PML RHS + lumped solve + Newmark)",

@@ -137,6 +137,7 @@ typedef struct {
     const int *src_count;      /* [nshots] */
     const int *src_nodes;      /* [sum of src_count] */
     const double *src_weight;  /* [sum of src_count][2]: u and w weights */
+    int kernel_variant;        /* 0 = default (1), 1 = packed values + 2-way gather, 2 = separate value arrays */
 } sfg_shot_desc;
 
 /* Creates a run with all state zeroed (elastic_wave.c:466-502). */

@@ -210,6 +210,8 @@ struct sfg_operator {
     int *rp = nullptr, *ci = nullptr;
     double *v64 = nullptr; // [7][nnz]
     float *v32 = nullptr;  // [7][nnz]
+    double *p64 = nullptr; // [nnz][8]: M, S1..S6, 0 (one wide load per nonzero)
+    float *p32 = nullptr;  // [nnz][8]
 };
 
 static int check_csr(int n, const int *p, const int *j, int m)
@@ -253,7 +255,8 @@ extern "C" int sfg_operator_create(sfg_operator **out, int n, const int *const c
     op->device = g_device;
     int rc = SFG_OK;
     if ((rc = dalloc(&op->rp, n + 1)) || (rc = dalloc(&op->ci, nnz)) ||
-        (rc = dalloc(&op->v64, (size_t)7 * nnz)) || (rc = dalloc(&op->v32, (size_t)7 * nnz))) {
+        (rc = dalloc(&op->v64, (size_t)7 * nnz)) || (rc = dalloc(&op->v32, (size_t)7 * nnz)) ||
+        (rc = dalloc(&op->p64, (size_t)8 * nnz)) || (rc = dalloc(&op->p32, (size_t)8 * nnz))) {
         sfg_operator_destroy(op);
         return rc;
     }
@@ -262,6 +265,13 @@ extern "C" int sfg_operator_create(sfg_operator **out, int n, const int *const c
     for (int m = 0; m < 7 && e == hipSuccess; ++m)
         e = hipMemcpy(op->v64 + (size_t)m * nnz, uxp[m], sizeof(double) * nnz, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = launch_to_f32(op->v64, op->v32, (size_t)7 * nnz, nullptr);
+    if (e == hipSuccess) {
+        std::vector<double> pk((size_t)8 * nnz, 0.0);
+        for (int k = 0; k < nnz; ++k)
+            for (int m = 0; m < 7; ++m) pk[(size_t)8 * k + m] = uxp[m][k];
+        e = hipMemcpy(op->p64, pk.data(), sizeof(double) * pk.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = launch_to_f32(op->p64, op->p32, (size_t)8 * nnz, nullptr);
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         sfg_operator_destroy(op);
@@ -280,6 +290,8 @@ extern "C" void sfg_operator_destroy(sfg_operator *op)
     dfree(op->ci);
     dfree(op->v64);
     dfree(op->v32);
+    dfree(op->p64);
+    dfree(op->p32);
     delete op;
 }
 
@@ -646,6 +658,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.rp = op->rp;
         a.ci = op->ci;
         for (int m = 0; m < 7; ++m) a.A[m] = vals + (size_t)m * op->nnz;
+        a.variant = d->kernel_variant == 2 ? 0 : 1;
         a.coef = r->cps ? (decltype(coef))r->coefp : coef;
         a.flag = r->cps ? r->flagp : md->flag;
         a.state = state;
@@ -671,9 +684,11 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.ctrl = r->ctrl;
     };
     if (r->prec == 0)
-        fill(r->a64, (double *)r->state, (double *)r->comp, op->v64, md->coef64, [](double v) { return v; });
+        fill(r->a64, (double *)r->state, (double *)r->comp, op->v64, md->coef64, [](double v) { return v; }),
+            r->a64.Ap = op->p64;
     else
-        fill(r->a32, (float *)r->state, (float *)r->comp, op->v32, md->coef32, [](double v) { return (float)v; });
+        fill(r->a32, (float *)r->state, (float *)r->comp, op->v32, md->coef32, [](double v) { return (float)v; }),
+            r->a32.Ap = op->p32;
 
     // one captured graph of guard_every steps followed by the energy guard
     if (r->use_graph) {

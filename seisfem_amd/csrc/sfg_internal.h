@@ -44,7 +44,9 @@ struct StepArgs {
     int xcd_remap;          // 1: contiguous block ranges per XCD
     int sp, cps, nbx;       // lane grouping, per-shot coefficients, row blocks (multi-run dispatch)
     const int *rp, *ci;     // union CSR pattern
-    const T *A[7];          // M, S1..S6 on the union pattern
+    const T *A[7];          // M, S1..S6 on the union pattern (one array each)
+    const T *Ap;            // the same values packed per nonzero: [nnz][8] = M, S1..S6, 0
+    int variant;            // 0: separate value arrays; 1: packed values, gather loop unrolled x2
     const T *coef;          // NCOEF x n (x S when coefficients are per shot)
     const int *flag;        // Dirichlet flags (exact mode), may be null otherwise
     T *state;               // [2][NFIELD][n][S]

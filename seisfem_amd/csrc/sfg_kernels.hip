@@ -57,7 +57,7 @@ __device__ __forceinline__ void finish_step(Ctrl *ctrl, int t, unsigned int tota
 }
 
 // One tile (logical block bx of rows, shot chunk by) of step t.
-template <class T, int PREC, int SP, bool CPS>
+template <class T, int PREC, int SP, bool CPS, bool PACKED>
 __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, int by)
 {
     constexpr bool EXACT = (PREC == 0);
@@ -83,11 +83,17 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         T mu1t = 0, mu1 = 0, mlx1 = 0, mlx2 = 0, mu2t = 0, mu2 = 0, mu3t = 0, mu3 = 0, mlx3 = 0, mlx4 = 0;
         T mw1t = 0, mw1 = 0, mly1 = 0, mly2 = 0, mw2t = 0, mw2 = 0, mw3t = 0, mw3 = 0, mly3 = 0, mly4 = 0;
         const int k0 = a.rp[row], k1 = a.rp[row + 1];
-        for (int k = k0; k < k1; ++k) {
+        auto nonzero = [&](int k) __attribute__((always_inline)) {
             const int j = a.ci[k];
-            const T am = a.A[0][k];
-            const T a1 = a.A[1][k], a2 = a.A[2][k], a3 = a.A[3][k];
-            const T a4 = a.A[4][k], a5 = a.A[5][k], a6 = a.A[6][k];
+            T am, a1, a2, a3, a4, a5, a6;
+            if constexpr (PACKED) {
+                const T *v = a.Ap + (size_t)k * 8;
+                am = v[0]; a1 = v[1]; a2 = v[2]; a3 = v[3]; a4 = v[4]; a5 = v[5]; a6 = v[6];
+            } else {
+                am = a.A[0][k];
+                a1 = a.A[1][k]; a2 = a.A[2][k]; a3 = a.A[3][k];
+                a4 = a.A[4][k]; a5 = a.A[5][k]; a6 = a.A[6][k];
+            }
             const T *x = in + (size_t)j * S + s;
             const T u1 = x[U1 * F], u2 = x[U2 * F], u3 = x[U3 * F];
             const T w1 = x[W1 * F], w2 = x[W2 * F], w3 = x[W3 * F];
@@ -114,6 +120,18 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             mw2t = mac<EXACT>(mw2t, am, w2t); mw2 = mac<EXACT>(mw2, am, w2);
             mw3t = mac<EXACT>(mw3t, am, w3t); mw3 = mac<EXACT>(mw3, am, w3);
             mly3 = mac<EXACT>(mly3, am, ly3); mly4 = mac<EXACT>(mly4, am, ly4);
+        };
+        if constexpr (PACKED) {
+            // two nonzeros per trip: the gathers of both are in flight together;
+            // each sum still runs in column order
+            int k = k0;
+            for (; k + 1 < k1; k += 2) {
+                nonzero(k);
+                nonzero(k + 1);
+            }
+            if (k < k1) nonzero(k);
+        } else {
+            for (int k = k0; k < k1; ++k) nonzero(k);
         }
 
         // ---- per-node coefficients --------------------------------------
@@ -241,14 +259,14 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
 
 }
 
-template <class T, int PREC, int SP, bool CPS>
+template <class T, int PREC, int SP, bool CPS, bool PACKED>
 __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a)
 {
     const int t = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int nb = gridDim.x * gridDim.y;
     unsigned int bid = blockIdx.y * gridDim.x + blockIdx.x;
     if (a.xcd_remap) bid = xcd_contiguous(bid, nb);
-    step_tile<T, PREC, SP, CPS>(a, t, (int)(bid % gridDim.x), (int)(bid / gridDim.x));
+    step_tile<T, PREC, SP, CPS, PACKED>(a, t, (int)(bid % gridDim.x), (int)(bid / gridDim.x));
     finish_step(a.ctrl, t, nb);
 }
 
@@ -273,12 +291,13 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
     const StepArgs<T> &a = args[r];
     const int local = (int)bid - tile_start[r];
     const int bx = local % a.nbx, by = local / a.nbx;
+    const bool pk = a.variant == 1;
     if (a.sp == 1) {
-        if (a.cps) step_tile<T, PREC, 1, true>(a, t, bx, by);
-        else step_tile<T, PREC, 1, false>(a, t, bx, by);
+        if (a.cps) { if (pk) step_tile<T, PREC, 1, true, true>(a, t, bx, by); else step_tile<T, PREC, 1, true, false>(a, t, bx, by); }
+        else { if (pk) step_tile<T, PREC, 1, false, true>(a, t, bx, by); else step_tile<T, PREC, 1, false, false>(a, t, bx, by); }
     } else {
-        if (a.cps) step_tile<T, PREC, 64, true>(a, t, bx, by);
-        else step_tile<T, PREC, 64, false>(a, t, bx, by);
+        if (a.cps) { if (pk) step_tile<T, PREC, 64, true, true>(a, t, bx, by); else step_tile<T, PREC, 64, true, false>(a, t, bx, by); }
+        else { if (pk) step_tile<T, PREC, 64, false, true>(a, t, bx, by); else step_tile<T, PREC, 64, false, false>(a, t, bx, by); }
     }
     finish_step(ctrl, t, nb);
 }
@@ -300,11 +319,10 @@ hipError_t launch_multi_step(int precision, const void *args_dev, const int *til
     return hipGetLastError();
 }
 
-template <class T, int PREC>
-static hipError_t launch_step_t(int sp, bool cps, const void *args, dim3 g, int block, hipStream_t st)
+template <class T, int PREC, bool PK>
+static hipError_t launch_step_v(int sp, bool cps, const StepArgs<T> &a, dim3 g, int block, hipStream_t st)
 {
-    const StepArgs<T> &a = *static_cast<const StepArgs<T> *>(args);
-#define SFG_L(SPV, C) hipLaunchKernelGGL((step_kernel<T, PREC, SPV, C>), g, dim3(block), 0, st, a)
+#define SFG_L(SPV, C) hipLaunchKernelGGL((step_kernel<T, PREC, SPV, C, PK>), g, dim3(block), 0, st, a)
     switch (sp) {
     case 1: if (cps) SFG_L(1, true); else SFG_L(1, false); break;
     case 4: if (cps) SFG_L(4, true); else SFG_L(4, false); break;
@@ -314,6 +332,14 @@ static hipError_t launch_step_t(int sp, bool cps, const void *args, dim3 g, int 
     }
 #undef SFG_L
     return hipGetLastError();
+}
+
+template <class T, int PREC>
+static hipError_t launch_step_t(int sp, bool cps, const void *args, dim3 g, int block, hipStream_t st)
+{
+    const StepArgs<T> &a = *static_cast<const StepArgs<T> *>(args);
+    if (a.variant == 1) return launch_step_v<T, PREC, true>(sp, cps, a, g, block, st);
+    return launch_step_v<T, PREC, false>(sp, cps, a, g, block, st);
 }
 
 hipError_t launch_step(int precision, int sp, bool cps, const void *args, int nbx, int nby,

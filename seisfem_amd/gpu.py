@@ -88,7 +88,7 @@ class Medium:
 class Run:
     def __init__(self, op, md, src_nodes, max_steps, rec_nodes, dt, f0, t0, amplitude=1.0e10,
                  angle_deg=90.0, precision=FP64, nsteps=None, block=0, shots_per_lane_group=0,
-                 guard_every=0, use_graph=True, xcd_remap=True, sources=None):
+                 guard_every=0, use_graph=True, xcd_remap=True, sources=None, kernel_variant=0):
         """sources: optional per-shot list of (nodes, weights[k][2]) distributed
         sources (u and w weights on each node) replacing the point forces."""
         L = lib()
@@ -110,6 +110,7 @@ class Run:
         d.block, d.shots_per_lane_group = block, shots_per_lane_group
         d.guard_every, d.use_graph = guard_every, int(bool(use_graph))
         d.xcd_remap = 0 if xcd_remap else -1
+        d.kernel_variant = int(kernel_variant)
         if sources is not None:
             if len(sources) != self.nshots:
                 raise ValueError("need one source description per shot")
