@@ -27,6 +27,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per step-kernel launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by a copy kernel of known size); written by the profiling runs
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
+
+
+def measured_traffic(workload_key):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f).get(workload_key)
+    except (OSError, ValueError):
+        return None
 
 
 def log(msg):
@@ -50,6 +61,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-acc", action="store_true")
+    ap.add_argument("--no-multi", action="store_true", help="skip config 4 part 2 (16 models in the same launch)")
     ap.add_argument("--quick", action="store_true", help="small mesh/steps for profiling runs")
     ap.add_argument("--block", type=int, default=0)
     args = ap.parse_args()
@@ -141,6 +153,52 @@ def main():
         acc = max(errs)
         log(f"accuracy: max rel-L2 vs FP64 over shots {idx.tolist()}: {acc:.3e}")
 
+    # config 4 part 2 in the same launch as part 1: 16 Q4 models of uneven size
+    multi = None
+    if not args.no_multi:
+        t_m = time.time()
+        models = workloads.config4b(step_range=(args.steps // 2, args.steps))
+        log(f"multi-model setup {time.time() - t_m:.1f}s: sizes {[c['size'] for c in models]}")
+        keep, runs = [], []
+        part1 = gpu.Run(op, md, case["src"], K, case["rec"], precision=prec, nsteps=nsteps, **kw)
+        runs.append(part1)
+        for c in models:
+            p = c["problem"]
+            mop = gpu.Operator(p["node_num"], p["csr"])
+            mmd = gpu.Medium(mop, p["c"], p["mass_lump"], p["dirichlet"], p["mpml"])
+            keep += [mop, mmd]
+            runs.append(gpu.Run(mop, mmd, c["src"], K, c["rec"], c["dt"], c["f0"], c["t0"], precision=prec,
+                                nsteps=c["nsteps"], block=kw.get("block", 0), shots_per_lane_group=1,
+                                kernel_variant=kw.get("kernel_variant", 0)))
+        grp = gpu.Multi(runs)
+        grp.advance(K)
+        ms_m = grp.last_ms_per_step
+        dof_m = dof_steps + sum(2.0 * c["problem"]["node_num"] * int(c["nsteps"][0]) for c in models)
+        # check two part-2 models against their own FP64 runs
+        errs = []
+        for qi in (0, len(models) - 1):
+            c = models[qi]
+            p = c["problem"]
+            u32, w32 = runs[1 + qi].seismograms()
+            r64m = gpu.Run(keep[2 * qi], keep[2 * qi + 1], c["src"], K, c["rec"], c["dt"], c["f0"], c["t0"],
+                           precision=gpu.FP64, nsteps=c["nsteps"])
+            r64m.advance(K)
+            u64m, w64m = r64m.seismograms()
+            r64m.close()
+            k = int(c["nsteps"][0])
+            errs.append(rel_l2(np.concatenate([u32[0, :k].ravel(), w32[0, :k].ravel()]),
+                               np.concatenate([u64m[0, :k].ravel(), w64m[0, :k].ravel()])))
+        multi = {"value": dof_m / (ms_m * K * 1e-3), "ms_per_step": ms_m, "models": len(models),
+                 "sizes": [list(c["size"]) for c in models], "max_rel_l2_checked": max(errs),
+                 "launch": "one kernel launch per step for part 1 (128 shots) + 16 models, hipGraph"}
+        log(f"multi-model launch: {ms_m:.3f} ms/step, {multi['value'] / 1e9:.2f} G DOF-steps/s, "
+            f"rel-L2 {max(errs):.2e}")
+        grp.close()
+        for r in runs:
+            r.close()
+        for h in keep:
+            h.close()
+
     # CPU baseline: the oracle (C restatement of the reference loop), bounded sample
     cpu = None
     if not args.no_cpu:
@@ -173,11 +231,13 @@ def main():
                    "nodes": N, "dof_per_shot": 2 * N, "shots": args.shots, "precision": args.precision,
                    "parallelism": "1 GPU", "tuned": best, "tuned_fp64": best64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"{args.precision}_{args.shots}x_q4_{args.nelem}"),
                      "kernel": "step_kernel (fused SpMM + M-PML RHS + lumped solve + Newmark)",
                      "ms_per_launch": ms_kernel, "bytes_per_launch": bytes_launch},
         "fp64_single": {"value": fp64_rate, "ms_per_step": ms64, "speedup_batched_vs_fp64": value / fp64_rate},
         "accuracy_max_rel_l2": acc,
+        "multi_model": multi,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

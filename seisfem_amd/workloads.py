@@ -69,6 +69,30 @@ def structured_case(type_code, nex, ney, edge, model_fn, pml, dt, f0, t0, steps_
                 xmax=xmax, ymax=ymax)
 
 
+def config4b(mpml_fn=None, nmodels=16, seed=41, dt=5.0e-4, step_range=(1500, 3000), sizes=None):
+    """16 independent Q4 models (dx = 10 m) with sizes drawn from
+    {100x100, 200x150, 300x200, 500x300} elements, each with its own seeded
+    4-layer isotropic model, dt 0.5 ms, 10 Hz Ricker, executed steps drawn
+    U[1500, 3000]; one source at 10 m depth mid-surface, 101 receivers."""
+    rng = np.random.default_rng(seed)
+    choices = sizes or [(100, 100), (200, 150), (300, 200), (500, 300)]
+    out = []
+    for m in range(nmodels):
+        nex, ney = choices[int(rng.integers(0, len(choices)))]
+        H = 10.0 * nex
+        mseed = int(rng.integers(0, 2 ** 31))
+
+        def model(depth, y, H=H, mseed=mseed):
+            return layered_model(depth, np.random.default_rng(mseed), 4, H, 1500.0, 4500.0, thomsen=False)
+
+        case = structured_case(host.Q4, nex, ney, 10.0, model, 20, dt, 10.0, 0.12, int(step_range[1]), 10.0,
+                               5.0 * ney, 101, 0.0, 10.0 * ney, mpml_fn)
+        case["nsteps"] = np.array([int(rng.integers(step_range[0], step_range[1] + 1))], np.int32)
+        case["size"] = (nex, ney)
+        out.append(case)
+    return out
+
+
 def _shape_grad_ref(type_code, r, s):
     """dphi/dr, dphi/ds of the reference element at (r, s) (same node order as
     shape/shape_{t3,t6,q4,q9}.c)."""

@@ -66,16 +66,18 @@ def _case(tc, nelem, steps, nshots=1):
     return workloads.small_q4(nelem=nelem, steps=steps, nshots=nshots, type_code=tc)
 
 
-@pytest.mark.parametrize("tc,nelem,steps,graph,guard,xcd", [
-    (host.Q4, 20, 150, True, 50, True), (host.Q4, 20, 150, False, 7, False), (host.Q9, 10, 120, True, 13, True),
-    (host.T6, 10, 120, True, 50, True), (host.T3, 20, 120, False, 50, True)])
-def test_fp64_loop_bit_identical(tc, nelem, steps, graph, guard, xcd):
+@pytest.mark.parametrize("tc,nelem,steps,graph,guard,xcd,variant", [
+    (host.Q4, 20, 150, True, 50, True, 1), (host.Q4, 20, 150, False, 7, False, 2),
+    (host.Q9, 10, 120, True, 13, True, 1), (host.Q9, 10, 120, True, 50, True, 2),
+    (host.T6, 10, 120, True, 50, True, 1), (host.T3, 20, 120, False, 50, True, 1)])
+def test_fp64_loop_bit_identical(tc, nelem, steps, graph, guard, xcd, variant):
     case = _case(tc, nelem, steps)
     prob = case["problem"]
     rec = np.arange(prob["node_num"], dtype=np.int32)  # every node is a receiver
-    say(f"FP64 loop {host.TYPE_NAMES[tc]} {nelem}x{nelem} steps={steps} graph={graph} guard={guard} xcd={xcd}")
+    say(f"FP64 loop {host.TYPE_NAMES[tc]} {nelem}x{nelem} steps={steps} graph={graph} guard={guard} xcd={xcd} "
+        f"variant={variant}")
     res = gpu.simulate(prob, case["src"], steps, rec, case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
-                       use_graph=graph, guard_every=guard, xcd_remap=xcd)
+                       use_graph=graph, guard_every=guard, xcd_remap=xcd, kernel_variant=variant)
     ref = oracle.oracle_masslump(prob, case["src"][0], steps + 2, rec, case["dt"], case["f0"], case["t0"])
     assert ref["status"] == steps
     assert np.max(np.abs(ref["seis_u"])) > 0
