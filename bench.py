@@ -57,13 +57,15 @@ def main():
     ap.add_argument("--shots", type=int, default=128)
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp64", "fp32", "fp32_kahan"])
     ap.add_argument("--fp64-steps", type=int, default=500)
-    ap.add_argument("--acc-shots", type=int, default=3)
+    ap.add_argument("--acc-shots", type=int, default=0, help="shots checked against FP64 (0 = all)")
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-acc", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip config 4 part 2 (16 models in the same launch)")
     ap.add_argument("--quick", action="store_true", help="small mesh/steps for profiling runs")
-    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--launch", default="",
+                    help="pin the batched launch shape 'shots_per_lane_group,block,variant,xcd_remap' (no tuning)")
+    ap.add_argument("--launch64", default="", help="pin the FP64 single-shot launch shape (same form)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -75,6 +77,10 @@ def main():
         args.acc_shots = 1
 
     from seisfem_amd import autotune, gpu, workloads
+
+    # measured copy bandwidth of this device (1 GiB buffers), reported beside the nominal peak
+    copy_gbs, copy_ms = gpu.stream_copy(1 << 30, 20)
+    log(f"stream copy 1 GiB: {copy_gbs:.0f} GB/s ({copy_ms:.3f} ms per copy)")
 
     t_setup = time.time()
     case = workloads.config4a(nshots=args.shots, nelem=args.nelem,
@@ -90,18 +96,27 @@ def main():
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
     base = dict(dt=case["dt"], f0=case["f0"], t0=case["t0"])
     # launch shape chosen by measurement on this problem (untimed)
-    if args.block:
-        best = dict(block=args.block)
+    if args.launch:
+        sp, blk, var, xcd = (int(v) for v in args.launch.split(","))
+        best = dict(shots_per_lane_group=sp, block=blk, kernel_variant=var, xcd_remap=bool(xcd))
     else:
         best, _ = autotune.tune(op, md, case["src"], case["rec"], precision=prec, steps=30, log=log, **base)
-    best64, _ = autotune.tune(op, md, case["src"][:1], case["rec"], precision=gpu.FP64, steps=30, log=log, **base)
+    if args.launch64:
+        sp, blk, var, xcd = (int(v) for v in args.launch64.split(","))
+        best64 = dict(shots_per_lane_group=sp, block=blk, kernel_variant=var, xcd_remap=bool(xcd))
+    else:
+        best64, _ = autotune.tune(op, md, case["src"][:1], case["rec"], precision=gpu.FP64, steps=30, log=log,
+                                  **base)
     log(f"tuned batched {best}; tuned fp64 single {best64}")
     kw = dict(base, **best)
     kw64 = dict(base, **best64)
 
-    # warmup: a separate run of the same shape (untimed)
-    warm = gpu.Run(op, md, case["src"], max(args.warmup, 1), case["rec"], precision=prec, **kw)
-    warm.advance(max(args.warmup, 1))
+    # warmup: a separate run of the same shape, every shot active (untimed)
+    W = max(args.warmup, 1)
+    warm = gpu.Run(op, md, case["src"], 2 * W, case["rec"], precision=prec, **dict(kw, guard_every=W))
+    warm.advance(W)
+    warm.advance(W)
+    ms_all_active = warm.last_ms_per_step
     warm.close()
 
     # timed whole job: K steps, shots stop at their own step counts
@@ -115,18 +130,18 @@ def main():
     value = dof_steps / (ms_step_dev * K * 1e-3)
     log(f"batched {args.precision}: {ms_step_dev:.3f} ms/step (device), wall {wall:.2f}s, "
         f"{value / 1e9:.2f} G DOF-steps/s")
+    # roofline of the step kernel over the same K launches: algorithmic bytes of
+    # launch t are those of the shots still active at t; the guard kernel (one
+    # per guard_every launches, ~0.1% of the time) is inside the event interval
+    active = [(nsteps > t).sum() for t in range(K)]
+    bytes_launch = float(np.mean([run.step_bytes(a) for a in active]))
+    achieved = bytes_launch / (ms_step_dev * 1e-3) / 1e9
+    bytes_full = run.step_bytes(args.shots)
     seis_u, seis_w = run.seismograms()
     run.close()
-
-    # roofline pass: all shots active, guard only at the end -> step-kernel time
-    Kr = min(400, K)
-    rr = gpu.Run(op, md, case["src"], Kr, case["rec"], precision=prec, **dict(kw, guard_every=Kr))
-    rr.advance(Kr)
-    ms_kernel = rr.last_ms_per_step
-    bytes_launch = rr.step_bytes(args.shots)
-    achieved = bytes_launch / (ms_kernel * 1e-3) / 1e9
-    rr.close()
-    log(f"step kernel: {ms_kernel:.4f} ms, {bytes_launch / 1e9:.3f} GB algorithmic -> {achieved:.0f} GB/s")
+    log(f"step kernel: {ms_step_dev:.4f} ms/launch, {bytes_launch / 1e9:.3f} GB algorithmic per launch "
+        f"-> {achieved:.0f} GB/s; all shots active: {ms_all_active:.4f} ms, "
+        f"{bytes_full / (ms_all_active * 1e-3) / 1e9:.0f} GB/s")
 
     # plain single-shot FP64 loop on the same mesh (the reference-exact path)
     r64 = gpu.Run(op, md, case["src"][:1], args.fp64_steps, case["rec"], precision=gpu.FP64, **kw64)
@@ -139,7 +154,8 @@ def main():
     # accuracy gate: FP32(+Kahan) seismograms vs the FP64 loop, full length, a few shots
     acc = None
     if not args.no_acc:
-        idx = np.linspace(0, args.shots - 1, args.acc_shots).astype(int)
+        nacc = args.acc_shots if 0 < args.acc_shots < args.shots else args.shots
+        idx = np.unique(np.linspace(0, args.shots - 1, nacc).astype(int))
         ra = gpu.Run(op, md, case["src"][idx], K, case["rec"], precision=gpu.FP64, nsteps=nsteps[idx], **base)
         ra.advance(K)
         u64, w64 = ra.seismograms()
@@ -151,7 +167,7 @@ def main():
             b = np.concatenate([u64[q, :k].ravel(), w64[q, :k].ravel()])
             errs.append(rel_l2(a, b))
         acc = max(errs)
-        log(f"accuracy: max rel-L2 vs FP64 over shots {idx.tolist()}: {acc:.3e}")
+        log(f"accuracy: max rel-L2 vs FP64 over {len(idx)} shots: {acc:.3e} (median {np.median(errs):.3e})")
 
     # config 4 part 2 in the same launch as part 1: 16 Q4 models of uneven size
     multi = None
@@ -174,22 +190,29 @@ def main():
         grp.advance(K)
         ms_m = grp.last_ms_per_step
         dof_m = dof_steps + sum(2.0 * c["problem"]["node_num"] * int(c["nsteps"][0]) for c in models)
-        # check two part-2 models against their own FP64 runs
+        # every part-2 model against its own FP64 run (the 16 FP64 runs share one launch too)
+        r64s = [gpu.Run(keep[2 * qi], keep[2 * qi + 1], c["src"], K, c["rec"], c["dt"], c["f0"], c["t0"],
+                        precision=gpu.FP64, nsteps=c["nsteps"]) for qi, c in enumerate(models)]
+        g64 = gpu.Multi(r64s)
+        g64.advance(K)
         errs = []
-        for qi in (0, len(models) - 1):
-            c = models[qi]
-            p = c["problem"]
+        for qi, c in enumerate(models):
             u32, w32 = runs[1 + qi].seismograms()
-            r64m = gpu.Run(keep[2 * qi], keep[2 * qi + 1], c["src"], K, c["rec"], c["dt"], c["f0"], c["t0"],
-                           precision=gpu.FP64, nsteps=c["nsteps"])
-            r64m.advance(K)
-            u64m, w64m = r64m.seismograms()
-            r64m.close()
+            u64m, w64m = r64s[qi].seismograms()
             k = int(c["nsteps"][0])
             errs.append(rel_l2(np.concatenate([u32[0, :k].ravel(), w32[0, :k].ravel()]),
                                np.concatenate([u64m[0, :k].ravel(), w64m[0, :k].ravel()])))
+        g64.close()
+        for r in r64s:
+            r.close()
+        # part-1 shots of the multi launch against the single-run result of the same steps
+        u1, w1 = part1.seismograms()
+        p1_same = bool(np.array_equal(u1, seis_u) and np.array_equal(w1, seis_w))
+        bytes_m = float(np.mean([grp.step_bytes(t) for t in range(K)]))
         multi = {"value": dof_m / (ms_m * K * 1e-3), "ms_per_step": ms_m, "models": len(models),
-                 "sizes": [list(c["size"]) for c in models], "max_rel_l2_checked": max(errs),
+                 "achieved_GBs": bytes_m / (ms_m * 1e-3) / 1e9,
+                 "sizes": [list(c["size"]) for c in models], "max_rel_l2_vs_fp64": max(errs),
+                 "part1_identical_to_single_run": p1_same,
                  "launch": "one kernel launch per step for part 1 (128 shots) + 16 models, hipGraph"}
         log(f"multi-model launch: {ms_m:.3f} ms/step, {multi['value'] / 1e9:.2f} G DOF-steps/s, "
             f"rel-L2 {max(errs):.2e}")
@@ -234,7 +257,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(f"{args.precision}_{args.shots}x_q4_{args.nelem}"),
                      "kernel": "step_kernel (fused SpMM + M-PML RHS + lumped solve + Newmark)",
-                     "ms_per_launch": ms_kernel, "bytes_per_launch": bytes_launch},
+                     "ms_per_launch": ms_step_dev, "bytes_per_launch": bytes_launch,
+                     "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs,
+                     "all_shots_active": {"ms_per_launch": ms_all_active, "bytes_per_launch": bytes_full,
+                                          "achieved": bytes_full / (ms_all_active * 1e-3) / 1e9}},
         "fp64_single": {"value": fp64_rate, "ms_per_step": ms64, "speedup_batched_vs_fp64": value / fp64_rate},
         "accuracy_max_rel_l2": acc,
         "multi_model": multi,

@@ -127,7 +127,8 @@ typedef struct {
     double amplitude;       /* 1e10 in the reference (elastic_wave.c:526) */
     double angle_deg;       /* 90 in the reference (elastic_wave.c:66) */
     int block;              /* threads per block, 0 = tuned default */
-    int shots_per_lane_group; /* SP in {1,4,16,64}, 0 = chosen from nshots */
+    int shots_per_lane_group; /* SP in {1,4,16,32,64}: consecutive shots sharing one row
+                                 in a wavefront (64/SP rows per wave); 0 = chosen from nshots */
     int guard_every;        /* energy-guard interval in steps, 0 = 50 */
     int use_graph;          /* 1 = replay captured hipGraphs of guard_every steps */
     int xcd_remap;          /* 0 = default (on): each XCD steps one contiguous row range; -1 = off */
@@ -202,6 +203,13 @@ void sfg_multi_destroy(sfg_multi *multi);
  * lambda_max from below, so apply a safety factor to dt_max. */
 int sfg_estimate_dt(const sfg_operator *op, const sfg_medium *md, int max_iter, double tol,
                     double *lambda_max, double *dt_max);
+
+/* Copy bandwidth of the device: `reps` timed launches of a plain copy kernel
+ * (stream_copy_kernel) over two buffers of `bytes` each (a positive multiple
+ * of 16).  *gbs = 2 * bytes / time per copy.  The yardstick for the HBM
+ * roofline and for scaling a profiler's byte counters; no counterpart in the
+ * reference. */
+int sfg_stream_copy(size_t bytes, int reps, double *gbs, double *ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,7 @@ def lib():
         "sfg_run_step_bytes": (_d, [_vp, _i]),
         "sfg_run_destroy": (None, [_vp]),
         "sfg_estimate_dt": (_i, [_vp, _vp, _i, _d, _pd, _pd]),
+        "sfg_stream_copy": (_i, [C.c_size_t, _i, _pd, _pd]),
         "sfg_multi_create": (_i, [C.POINTER(_vp), C.POINTER(_vp), _i]),
         "sfg_multi_advance": (_i, [_vp, _i]),
         "sfg_multi_last_ms_per_step": (_d, [_vp]),

@@ -22,7 +22,7 @@ def candidates(nshots):
     elif nshots <= 16:
         sps = [16, 64]
     else:
-        sps = [64]
+        sps = [16, 32, 64]
     return list(itertools.product(sps, [64, 128, 256], [1, 2], [True, False]))
 
 

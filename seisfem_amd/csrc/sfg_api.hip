@@ -499,17 +499,15 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     // lane grouping: shots per lane group and the padded shot stride
     int sp = d->shots_per_lane_group;
     if (sp == 0) sp = d->nshots <= 1 ? 1 : d->nshots <= 4 ? 4 : d->nshots <= 16 ? 16 : 64;
-    if (sp != 1 && sp != 4 && sp != 16 && sp != 64) {
+    if (sp != 1 && sp != 4 && sp != 16 && sp != 32 && sp != 64) {
         run_free(r);
-        return fail(SFG_E_ARG, "run: shots_per_lane_group must be 1, 4, 16 or 64");
+        return fail(SFG_E_ARG, "run: shots_per_lane_group must be 1, 4, 16, 32 or 64");
     }
-    if (sp < 64 && d->nshots > sp) {
-        run_free(r);
-        return fail(SFG_E_ARG, "run: %d shots do not fit a lane group of %d", d->nshots, sp);
-    }
+    // shots are padded to whole lane groups; each group of sp shots is one
+    // column of tiles (blockIdx.y)
     r->SP = sp;
-    r->S = sp < 64 ? sp : 64 * ((d->nshots + 63) / 64);
-    r->nby = sp < 64 ? 1 : r->S / 64;
+    r->S = sp * ((d->nshots + sp - 1) / sp);
+    r->nby = r->S / sp;
     if ((size_t)n * r->S >= (size_t)1 << 31) {
         run_free(r);
         return fail(SFG_E_ARG, "run: node_num * padded shots must stay below 2^31");
@@ -1088,4 +1086,38 @@ extern "C" void sfg_multi_destroy(sfg_multi *m)
     if (!m) return;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     multi_free(m);
+}
+
+// ---------------------------------------------------------------------------
+// HBM yardstick (no counterpart in the reference)
+
+extern "C" int sfg_stream_copy(size_t bytes, int reps, double *gbs, double *ms_per_copy)
+{
+    g_err.clear();
+    if (bytes < 16 || bytes % 16 || reps <= 0 || !gbs || !ms_per_copy)
+        return fail(SFG_E_ARG, "stream_copy: bytes must be a positive multiple of 16, reps > 0");
+    char *a = nullptr, *b = nullptr;
+    int rc = dalloc(&a, bytes);
+    if (!rc) rc = dalloc(&b, bytes);
+    if (rc) { dfree(a); dfree(b); return rc; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_fill_zero(a, bytes, nullptr);
+    // two untimed copies, then `reps` timed ones
+    for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_stream_copy(a, b, bytes / 16, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_stream_copy(a, b, bytes / 16, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(a);
+    dfree(b);
+    if (e != hipSuccess) return fail(SFG_E_HIP, "stream_copy: %s", hipGetErrorString(e));
+    *ms_per_copy = (double)ms / reps;
+    *gbs = 2.0 * (double)bytes / (*ms_per_copy * 1e-3) / 1e9;
+    return SFG_OK;
 }

@@ -96,5 +96,6 @@ hipError_t launch_elastic_apply(int n, const int *rp, const int *ci, const doubl
                                 const double *coef, const int *flag, const double *x, double *y,
                                 double *sumsq, hipStream_t st);
 hipError_t launch_scale(size_t n, double *x, const double *sumsq, hipStream_t st);
+hipError_t launch_stream_copy(const void *src, void *dst, size_t n16, hipStream_t st);
 
 } // namespace sfg

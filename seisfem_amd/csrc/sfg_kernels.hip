@@ -9,9 +9,10 @@
 //
 // Lane mapping: SP lanes of a wavefront share one row and hold consecutive
 // shots (state layout [field][node][shot], shots innermost), so a wave covers
-// 64/SP rows.  SP = 1 is the single-shot path (one row per lane); SP = 64
-// gives one row per wave with the matrix row read once as scalars and
-// broadcast to 64 shots.
+// 64/SP rows; shot groups of SP are the grid's y dimension.  SP = 1 is the
+// single-shot path (one row per lane); SP = 64 gives one row per wave with
+// the matrix row read once as scalars and broadcast to 64 shots; SP = 16 or
+// 32 trade that broadcast for a smaller per-node footprint in L2.
 //
 // Precision modes:
 //   0  FP64, bit-identical to the reference: every sum is sequential in
@@ -327,6 +328,7 @@ static hipError_t launch_step_v(int sp, bool cps, const StepArgs<T> &a, dim3 g, 
     case 1: if (cps) SFG_L(1, true); else SFG_L(1, false); break;
     case 4: if (cps) SFG_L(4, true); else SFG_L(4, false); break;
     case 16: if (cps) SFG_L(16, true); else SFG_L(16, false); break;
+    case 32: if (cps) SFG_L(32, true); else SFG_L(32, false); break;
     case 64: if (cps) SFG_L(64, true); else SFG_L(64, false); break;
     default: return hipErrorInvalidValue;
     }
@@ -675,6 +677,21 @@ hipError_t launch_elastic_apply(int n, const int *rp, const int *ci, const doubl
 hipError_t launch_scale(size_t n, double *x, const double *sumsq, hipStream_t st)
 {
     hipLaunchKernelGGL(scale_kernel, dim3(1024), dim3(256), 0, st, n, x, sumsq);
+    return hipGetLastError();
+}
+
+// Plain 16-byte-per-lane copy: the yardstick for the HBM roofline and for
+// scaling the byte counters of a profiler (it moves exactly 2 x n x 16 bytes).
+__global__ __launch_bounds__(256) void stream_copy_kernel(const float4 *__restrict__ src,
+                                                          float4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_stream_copy(const void *src, void *dst, size_t n16, hipStream_t st)
+{
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(8192), dim3(256), 0, st, (const float4 *)src, (float4 *)dst, n16);
     return hipGetLastError();
 }
 

@@ -209,6 +209,14 @@ class Multi:
             pass
 
 
+def stream_copy(nbytes=1 << 30, reps=20):
+    """(GB/s, ms per copy) of a plain device copy of `nbytes` (read + write counted)."""
+    gbs = C.c_double(0.0)
+    ms = C.c_double(0.0)
+    check(lib().sfg_stream_copy(int(nbytes), int(reps), C.byref(gbs), C.byref(ms)), "sfg_stream_copy")
+    return gbs.value, ms.value
+
+
 def estimate_dt(op, md, max_iter=2000, tol=1e-9):
     """(lambda_max, dt_max, iterations) of M_L^{-1} K by FP64 power iteration on the GPU."""
     lam = C.c_double(0.0)

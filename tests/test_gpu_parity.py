@@ -105,7 +105,7 @@ def test_example6_on_gpu_character_identical():
         os.path.join(out, "wavefield_w_shot_1.txt"))
 
 
-@pytest.mark.parametrize("nshots,sp", [(3, 4), (11, 16), (70, 64)])
+@pytest.mark.parametrize("nshots,sp", [(3, 4), (11, 16), (70, 64), (37, 16), (45, 32), (5, 1)])
 def test_multishot_fp64_each_shot_bit_identical(nshots, sp):
     case = _case(host.Q4, 14, 90, nshots=nshots)
     prob = case["problem"]
