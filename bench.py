@@ -176,7 +176,10 @@ def main():
         models = workloads.config4b(step_range=(args.steps // 2, args.steps))
         log(f"multi-model setup {time.time() - t_m:.1f}s: sizes {[c['size'] for c in models]}")
         keep, runs = [], []
-        part1 = gpu.Run(op, md, case["src"], K, case["rec"], precision=prec, nsteps=nsteps, **kw)
+        # the multi-run kernel takes 1- or 64-shot lane groups; a lane's arithmetic does not
+        # depend on the grouping, so part 1 stays bit-identical to the single run
+        part1 = gpu.Run(op, md, case["src"], K, case["rec"], precision=prec, nsteps=nsteps,
+                        **dict(kw, shots_per_lane_group=64))
         runs.append(part1)
         for c in models:
             p = c["problem"]

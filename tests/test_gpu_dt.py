@@ -83,3 +83,10 @@ def test_loop_stability_limit():
         bad.advance(600)
     assert ei.value.code == -6
     bad.close()
+
+
+def test_stream_copy_yardstick():
+    say("stream copy 256 MiB")
+    gbs, ms = gpu.stream_copy(256 << 20, 5)
+    # 2 x 256 MiB moved per copy: a device copy is well above PCIe rates and below 2x the HBM peak
+    assert 500.0 < gbs < 16000.0 and ms > 0.0

@@ -192,8 +192,9 @@ def test_multi_run_one_launch_per_step_matches_separate_runs():
         if nshots > 1:
             nsteps = np.random.default_rng(1).integers(30, 91, nshots).astype(np.int32)
             nsteps[0] = 90
+        # one launch has one block size: the runs of a group share it
         run = gpu.Run(op, md, case["src"], 90, case["rec"], case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
-                      nsteps=nsteps, guard_every=25)
+                      nsteps=nsteps, guard_every=25, block=256)
         cases.append((case, nsteps))
         runs.append(run)
         keep += [op, md]

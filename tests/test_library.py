@@ -43,3 +43,12 @@ def test_host_seismic_source_matches_formula():
     f0, t0, t = 10.0, 0.1, 0.137
     a = (math.pi * f0 * (t - t0)) ** 2
     assert abs(L.seismic_source(f0, t0, 1e10, t) - 1e10 * (1 - 2 * a) * math.exp(-a)) < 1e-3
+
+
+def test_stream_copy_rejects_bad_sizes_before_touching_the_device():
+    import pytest
+    from seisfem_amd import NativeError, gpu
+    for nbytes, reps in ((15, 1), (0, 1), (1024, 0)):
+        with pytest.raises(NativeError) as e:
+            gpu.stream_copy(nbytes, reps)
+        assert e.value.code == -1
