@@ -117,9 +117,12 @@ def _shape_grad_ref(type_code, r, s):
 def explosive_source(type_code, element_node, x, y, xs, ys, moment=1.0):
     """Equivalent nodal forces of an isotropic moment tensor M0*I at (xs, ys):
     f_i = M0 * grad(phi_i)(xs, ys) on the nodes of the element holding the point
-    (the weak form of -div(M delta)).  Returns (nodes, weights [k][2]) with the
-    u (x) and w (y) weights.  Not in the reference, whose only source is a point
-    force (time_evolution/elastic_wave.c:585, 651)."""
+    (the weak form of -div(M delta)).  The gradient jumps across element edges,
+    so a point on an edge or a vertex takes the mean over the elements sharing
+    it (mirror-symmetric meshes then give mirror-symmetric forces).  Returns
+    (nodes, weights [k][2]) with the u (x) and w (y) weights.  Not in the
+    reference, whose only source is a point force
+    (time_evolution/elastic_wave.c:585, 651)."""
     o = host.element_order(type_code)
     en = np.asarray(element_node, np.int64).reshape(-1, o) - 1
     quad = type_code in (host.Q4, host.Q9)
@@ -127,6 +130,8 @@ def explosive_source(type_code, element_node, x, y, xs, ys, moment=1.0):
     cx, cy = x[en[:, :nv]], y[en[:, :nv]]
     cand = np.flatnonzero((cx.min(1) <= xs + 1e-9) & (cx.max(1) >= xs - 1e-9) &
                           (cy.min(1) <= ys + 1e-9) & (cy.max(1) >= ys - 1e-9))
+    acc = {}
+    hits = 0
     for e in cand:
         nd = en[e]
         vx, vy = x[nd[:nv]], y[nd[:nv]]
@@ -154,8 +159,14 @@ def explosive_source(type_code, element_node, x, y, xs, ys, moment=1.0):
         inv = np.linalg.inv(J)  # [dr/dx dr/dy; ds/dx ds/dy]
         gx = dr * inv[0, 0] + ds * inv[1, 0]
         gy = dr * inv[0, 1] + ds * inv[1, 1]
-        return nd.astype(np.int32), moment * np.stack([gx, gy], axis=1)
-    raise ValueError("source point outside the mesh")
+        for k, node in enumerate(nd):
+            a = acc.setdefault(int(node), np.zeros(2))
+            a += (gx[k], gy[k])
+        hits += 1
+    if not hits:
+        raise ValueError("source point outside the mesh")
+    nodes = np.array(sorted(acc), np.int32)
+    return nodes, moment * np.array([acc[int(i)] for i in nodes]) / hits
 
 
 def topography_t6(nex, ney, edge, hills, rng=None):

@@ -39,3 +39,17 @@ def test_topography_mesh_is_straight_sided_and_follows_the_surface():
     area = 0.5 * ((x[v[:, 1]] - x[v[:, 0]]) * (y[v[:, 2]] - y[v[:, 0]]) -
                   (x[v[:, 2]] - x[v[:, 0]]) * (y[v[:, 1]] - y[v[:, 0]]))
     assert np.all(np.abs(area) > 1.0)
+
+
+@pytest.mark.parametrize("tc", [host.Q4, host.Q9])
+def test_explosive_weights_mirror_symmetric_on_an_element_edge(tc):
+    """A source on the mesh's mirror line (an element edge) gets forces whose
+    x part is even and y part odd about that line: the one-sided gradients of
+    the two elements sharing the edge are averaged."""
+    en, x, y = host.mesh_structured(tc, 6, 8, 0.0, 60.0, 0.0, 80.0)
+    nodes, w = workloads.explosive_source(tc, en, x, y, 35.0, 40.0)
+    assert np.any(np.isclose(y[nodes], 30.0)) and np.any(np.isclose(y[nodes], 50.0))  # both elements
+    f = {(round(x[n], 6), round(y[n], 6)): w[k] for k, n in enumerate(nodes)}
+    for (xx, yy), (fx, fy) in f.items():
+        gx, gy = f[(xx, round(80.0 - yy, 6))]
+        assert abs(fx - gx) < 1e-12 and abs(fy + gy) < 1e-12
