@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp64", "fp32", "fp32_kahan"])
     ap.add_argument("--fp64-steps", type=int, default=500)
     ap.add_argument("--acc-shots", type=int, default=0, help="shots checked against FP64 (0 = all)")
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="host time of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-acc", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip config 4 part 2 (16 models in the same launch)")
@@ -73,7 +73,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.quick:
-        args.nelem, args.steps, args.fp64_steps, args.cpu_steps = 100, 200, 100, 10
+        args.nelem, args.steps, args.fp64_steps, args.cpu_seconds = 100, 200, 100, 2.0
         args.acc_shots = 1
 
     from seisfem_amd import autotune, gpu, workloads
@@ -230,13 +230,20 @@ def main():
     if not args.no_cpu:
         import oracle
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        ts = time.time()
-        r = oracle.oracle_masslump(prob, int(case["src"][0]), args.cpu_steps + 2, case["rec"], case["dt"],
-                                   case["f0"], case["t0"])
-        tc = time.time() - ts
-        cpu = {"value": 2.0 * N * args.cpu_steps / tc, "unit": "DOF-steps/s", "cores": threads, "kind": "port",
-               "sample": f"FP64 single shot, {args.cpu_steps} steps of the same Q4 {args.nelem}x{args.nelem} "
-                         f"mesh ({tc:.1f}s), 44 csr_matvec calls per step as the reference issues them"}
+
+        def cpu_run(k):
+            ts = time.time()
+            oracle.oracle_masslump(prob, int(case["src"][0]), k + 2, case["rec"], case["dt"], case["f0"], case["t0"])
+            return time.time() - ts
+
+        # size the sample to about args.cpu_seconds of host work from a short calibration run
+        t_cal = cpu_run(20)
+        ksteps = int(min(K, max(20, args.cpu_seconds * 20 / max(t_cal, 1e-6))))
+        tc = cpu_run(ksteps)
+        cpu = {"value": 2.0 * N * ksteps / tc, "unit": "DOF-steps/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 single shot, the first {ksteps} steps of shot 0 on the same Q4 {args.nelem}x"
+                         f"{args.nelem} mesh ({tc:.1f}s wall incl. setup), 44 csr_matvec calls per step as the "
+                         f"reference issues them, OpenMP over rows"}
         log(f"cpu baseline: {cpu['value'] / 1e6:.1f} M DOF-steps/s on {threads} threads ({tc:.1f}s)")
 
     out = {

@@ -138,7 +138,10 @@ typedef struct {
     const int *src_count;      /* [nshots] */
     const int *src_nodes;      /* [sum of src_count] */
     const double *src_weight;  /* [sum of src_count][2]: u and w weights */
-    int kernel_variant;        /* 0 = default (1), 1 = packed values + 2-way gather, 2 = separate value arrays */
+    int kernel_variant;        /* 0 = default (1), 1 = packed values + 2-way gather, 2 = separate value
+                                  arrays, 3 = 1 + streaming hints on data used once per step,
+                                  4 = 3 + own-row fields kept from the diagonal gather.  Same results
+                                  bit for bit; a multi-run launch runs 3 and 4 as 1 */
 } sfg_shot_desc;
 
 /* Creates a run with all state zeroed (elastic_wave.c:466-502). */

@@ -23,7 +23,7 @@ def candidates(nshots):
         sps = [16, 64]
     else:
         sps = [16, 32, 64]
-    return list(itertools.product(sps, [64, 128, 256], [1, 2], [True, False]))
+    return list(itertools.product(sps, [64, 128, 256], [1, 2, 3, 4], [True, False]))
 
 
 def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None):

@@ -390,7 +390,7 @@ struct sfg_run {
     double *src_wt = nullptr;
     double *wavelet = nullptr, *seis_u = nullptr, *seis_w = nullptr, *energy = nullptr;
     hipStream_t stream = nullptr;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr}; // guard_every steps + guard, by parity of the first step
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int executed = 0;
     double last_ms = 0.0;
@@ -401,7 +401,8 @@ struct sfg_run {
 
 static void run_free(sfg_run *r)
 {
-    if (r->graph) (void)hipGraphExecDestroy(r->graph);
+    for (hipGraphExec_t g : r->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     if (r->ev0) (void)hipEventDestroy(r->ev0);
     if (r->ev1) (void)hipEventDestroy(r->ev1);
     if (r->stream) (void)hipStreamDestroy(r->stream);
@@ -430,10 +431,11 @@ extern "C" void sfg_run_destroy(sfg_run *r)
     run_free(r);
 }
 
-static int launch_one_step(sfg_run *r)
+// slot = parity of the step this launch executes
+static int launch_one_step(sfg_run *r, int slot)
 {
     const void *args = (r->prec == 0) ? (const void *)&r->a64 : (const void *)&r->a32;
-    HIPCHK(launch_step(r->prec, r->SP, r->cps, args, r->nbx, r->nby, r->block, r->stream));
+    HIPCHK(launch_step(r->prec, r->SP, r->cps, args, r->nbx, r->nby, r->block, slot, r->stream));
     return SFG_OK;
 }
 
@@ -446,6 +448,28 @@ static int launch_guard_now(sfg_run *r, const Ctrl *ctrl = nullptr, hipStream_t 
     return SFG_OK;
 }
 
+// Captures what `launches` enqueues on `st` into an instantiated graph.
+template <class F>
+static int capture_graph(hipStream_t st, hipGraphExec_t *out, F launches)
+{
+    hipGraph_t g = nullptr;
+    int rc = SFG_OK;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = launches();
+        hipError_t e2 = hipStreamEndCapture(st, &g);
+        if (rc == SFG_OK && e2 != hipSuccess) rc = fail(SFG_E_HIP, "graph capture: %s", hipGetErrorString(e2));
+    } else {
+        rc = fail(SFG_E_HIP, "graph capture: %s", hipGetErrorString(e));
+    }
+    if (rc == SFG_OK) {
+        e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = fail(SFG_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
 static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medium *const *media, int nmedia,
                            const sfg_shot_desc *d)
 {
@@ -453,6 +477,8 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (!out || !op || !media || nmedia <= 0 || !d) return fail(SFG_E_ARG, "run: bad arguments");
     *out = nullptr;
     if (d->precision < 0 || d->precision > 2) return fail(SFG_E_ARG, "run: unknown precision %d", d->precision);
+    if (d->kernel_variant < 0 || d->kernel_variant > 4)
+        return fail(SFG_E_ARG, "run: unknown kernel_variant %d", d->kernel_variant);
     if (d->nshots <= 0 || !d->src_node) return fail(SFG_E_ARG, "run: need >= 1 shot with a source node");
     if (nmedia != 1 && nmedia != d->nshots) return fail(SFG_E_ARG, "run: need one medium or one per shot");
     for (int k = 0; k < nmedia; ++k) {
@@ -656,7 +682,9 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.rp = op->rp;
         a.ci = op->ci;
         for (int m = 0; m < 7; ++m) a.A[m] = vals + (size_t)m * op->nnz;
-        a.variant = d->kernel_variant == 2 ? 0 : 1;
+        // descriptor variant -> kernel variant (see step_tile)
+        static const int kv[5] = {1, 1, 0, 2, 3};
+        a.variant = kv[d->kernel_variant];
         a.coef = r->cps ? (decltype(coef))r->coefp : coef;
         a.flag = r->cps ? r->flagp : md->flag;
         a.state = state;
@@ -688,23 +716,15 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         fill(r->a32, (float *)r->state, (float *)r->comp, op->v32, md->coef32, [](double v) { return (float)v; }),
             r->a32.Ap = op->p32;
 
-    // one captured graph of guard_every steps followed by the energy guard
+    // captured graphs of guard_every steps followed by the energy guard: one
+    // per parity of the first step (a second one only when guard_every is odd)
     if (r->use_graph) {
-        hipGraph_t g = nullptr;
-        e = hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-            for (int k = 0; k < r->guard_every && rc == SFG_OK; ++k) rc = launch_one_step(r);
-            if (rc == SFG_OK) rc = launch_guard_now(r);
-            hipError_t e2 = hipStreamEndCapture(r->stream, &g);
-            if (rc == SFG_OK && e2 != hipSuccess) rc = fail(SFG_E_HIP, "graph capture: %s", hipGetErrorString(e2));
-        } else {
-            rc = fail(SFG_E_HIP, "graph capture: %s", hipGetErrorString(e));
-        }
-        if (rc == SFG_OK) {
-            e = hipGraphInstantiate(&r->graph, g, nullptr, nullptr, 0);
-            if (e != hipSuccess) rc = fail(SFG_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
-        }
-        if (g) (void)hipGraphDestroy(g);
+        for (int p = 0; p < (r->guard_every % 2 ? 2 : 1) && rc == SFG_OK; ++p)
+            rc = capture_graph(r->stream, &r->graph[p], [&]() {
+                int c = SFG_OK;
+                for (int k = 0; k < r->guard_every && c == SFG_OK; ++k) c = launch_one_step(r, (p + k) & 1);
+                return c == SFG_OK ? launch_guard_now(r) : c;
+            });
         if (rc != SFG_OK) {
             run_free(r);
             return rc;
@@ -742,7 +762,8 @@ static int check_guard(sfg_run *r, int upto_step, const Ctrl *ctrl = nullptr)
     Ctrl h;
     HIPCHK(hipMemcpy(&h, ctrl ? ctrl : r->ctrl, sizeof h, hipMemcpyDeviceToHost));
     if (h.err & 1) return fail(SFG_E_ZERO_MASS, "zero lumped mass met (outputs of those nodes left unchanged)");
-    if (h.step != r->executed) return fail(SFG_E_HIP, "device step counter %d != host %d", h.step, r->executed);
+    const int dev_steps = std::max(h.step[0], h.step[1]);
+    if (dev_steps != r->executed) return fail(SFG_E_HIP, "device step counter %d != host %d", dev_steps, r->executed);
     return SFG_OK;
 }
 
@@ -755,12 +776,13 @@ extern "C" int sfg_run_advance(sfg_run *r, int nsteps)
     const int steps = todo;
     HIPCHK(hipEventRecord(r->ev0, r->stream));
     while (todo > 0) {
-        if (r->graph && todo >= r->guard_every && r->executed % r->guard_every == 0) {
-            HIPCHK(hipGraphLaunch(r->graph, r->stream));
+        hipGraphExec_t g = r->graph[r->executed & 1];
+        if (g && todo >= r->guard_every && r->executed % r->guard_every == 0) {
+            HIPCHK(hipGraphLaunch(g, r->stream));
             r->executed += r->guard_every;
             todo -= r->guard_every;
         } else {
-            int rc = launch_one_step(r);
+            int rc = launch_one_step(r, r->executed & 1);
             if (rc) return rc;
             r->executed += 1;
             todo -= 1;
@@ -915,7 +937,7 @@ struct sfg_multi {
     void *args = nullptr;       // device StepArgs<T>[nruns]
     int *tile_start = nullptr;  // device [nruns + 1]
     hipStream_t stream = nullptr;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr}; // by parity of the first step
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
 };
@@ -923,7 +945,8 @@ struct sfg_multi {
 static void multi_free(sfg_multi *m)
 {
     for (sfg_run *r : m->runs) r->in_multi = false;
-    if (m->graph) (void)hipGraphExecDestroy(m->graph);
+    for (hipGraphExec_t g : m->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
     if (m->ev1) (void)hipEventDestroy(m->ev1);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -933,9 +956,9 @@ static void multi_free(sfg_multi *m)
     delete m;
 }
 
-static int multi_launch_step(sfg_multi *m)
+static int multi_launch_step(sfg_multi *m, int slot)
 {
-    HIPCHK(launch_multi_step(m->prec, m->args, m->tile_start, (int)m->runs.size(), m->ntiles, m->block, 1,
+    HIPCHK(launch_multi_step(m->prec, m->args, m->tile_start, (int)m->runs.size(), m->ntiles, m->block, 1, slot,
                              m->stream));
     return SFG_OK;
 }
@@ -1006,22 +1029,13 @@ extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns
         multi_free(m);
         return fail(SFG_E_HIP, "multi setup: %s", hipGetErrorString(e));
     }
-    // graph: guard_every launches, then every run's energy guard
-    hipGraph_t g = nullptr;
-    e = hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-        for (int k = 0; k < m->guard_every && rc == SFG_OK; ++k) rc = multi_launch_step(m);
-        if (rc == SFG_OK) rc = multi_launch_guards(m);
-        hipError_t e2 = hipStreamEndCapture(m->stream, &g);
-        if (rc == SFG_OK && e2 != hipSuccess) rc = fail(SFG_E_HIP, "multi graph capture: %s", hipGetErrorString(e2));
-    } else {
-        rc = fail(SFG_E_HIP, "multi graph capture: %s", hipGetErrorString(e));
-    }
-    if (rc == SFG_OK) {
-        e = hipGraphInstantiate(&m->graph, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) rc = fail(SFG_E_HIP, "multi graph instantiate: %s", hipGetErrorString(e));
-    }
-    if (g) (void)hipGraphDestroy(g);
+    // graphs: guard_every launches, then every run's energy guard (per start parity)
+    for (int p = 0; p < (m->guard_every % 2 ? 2 : 1) && rc == SFG_OK; ++p)
+        rc = capture_graph(m->stream, &m->graph[p], [&]() {
+            int c = SFG_OK;
+            for (int k = 0; k < m->guard_every && c == SFG_OK; ++k) c = multi_launch_step(m, (p + k) & 1);
+            return c == SFG_OK ? multi_launch_guards(m) : c;
+        });
     if (rc != SFG_OK) {
         multi_free(m);
         return rc;
@@ -1039,12 +1053,13 @@ extern "C" int sfg_multi_advance(sfg_multi *m, int nsteps)
     const int steps = todo;
     HIPCHK(hipEventRecord(m->ev0, m->stream));
     while (todo > 0) {
-        if (todo >= m->guard_every && m->executed % m->guard_every == 0) {
-            HIPCHK(hipGraphLaunch(m->graph, m->stream));
+        hipGraphExec_t g = m->graph[m->executed & 1];
+        if (g && todo >= m->guard_every && m->executed % m->guard_every == 0) {
+            HIPCHK(hipGraphLaunch(g, m->stream));
             m->executed += m->guard_every;
             todo -= m->guard_every;
         } else {
-            int rc = multi_launch_step(m);
+            int rc = multi_launch_step(m, m->executed & 1);
             if (rc) return rc;
             m->executed += 1;
             todo -= 1;

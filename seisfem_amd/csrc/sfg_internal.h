@@ -27,13 +27,23 @@ enum Coef : int {
 };
 
 // Device-side control block: the step counter lets one captured graph replay
-// any chunk of steps; the last block of each step kernel advances it.
+// any chunk of steps.  A step kernel launched with slot p reads its step t from
+// step[p] and one thread of it stores t + 1 into step[p ^ 1], which the next
+// launch (slot p ^ 1) reads; launches on one stream run one after the other,
+// so no block of a launch can see its own store.  Executed steps =
+// max(step[0], step[1]).
 struct Ctrl {
-    int step;           // number of executed steps
-    unsigned int done;  // blocks finished in the current step kernel
+    int step[2];
     int err;            // bit 0: zero lumped mass met
     int pad;
 };
+
+__device__ __forceinline__ int executed_steps(const Ctrl *c)
+{
+    const int a = __hip_atomic_load(&c->step[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int b = __hip_atomic_load(&c->step[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return a > b ? a : b;
+}
 
 template <class T>
 struct StepArgs {
@@ -67,10 +77,11 @@ struct StepArgs {
 };
 
 // Launchers (defined in sfg_kernels.hip).  precision: 0 exact FP64, 1 FP32, 2 FP32+Kahan.
+// slot: parity of the step being launched (see Ctrl).
 hipError_t launch_step(int precision, int sp, bool coef_per_shot, const void *args, int nblk_x,
-                       int nblk_y, int block, hipStream_t st);
+                       int nblk_y, int block, int slot, hipStream_t st);
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
-                             int block, int xcd_remap, hipStream_t st);
+                             int block, int xcd_remap, int slot, hipStream_t st);
 hipError_t launch_guard(int precision, const void *state, int n, int S, size_t fstride,
                         const Ctrl *ctrl, double *energy, int nslots, int every, hipStream_t st);
 hipError_t launch_gather_uw(int precision, const void *state, int n, int S, size_t fstride,

@@ -32,6 +32,24 @@ __device__ __forceinline__ T mac(T acc, T a, T x)
     else return __builtin_fmaf(a, x, acc);
 }
 
+// Loads and stores of data no other lane of this step touches again (a row's
+// own accelerations, the Kahan terms, every output) may carry the streaming
+// (nontemporal) hint, so the lines they occupy do not push the neighbours'
+// fields, which nine rows gather, out of L2.
+template <bool NT, class T>
+__device__ __forceinline__ T ld_once(const T *p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <bool NT, class T>
+__device__ __forceinline__ void st_once(T *p, T v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; this map makes each
 // XCD walk one contiguous range of logical blocks (shot chunk, row block), so
 // neighbouring rows share that XCD's L2.  A bijection on [0, nb) for any nb.
@@ -42,27 +60,31 @@ __device__ __forceinline__ unsigned int xcd_contiguous(unsigned int bid, unsigne
     return x * q + (x < r ? x : r) + l;
 }
 
-// Publishes the end of one step kernel: the last block to finish advances
-// the device step counter (every block has read it before it counts itself).
-__device__ __forceinline__ void finish_step(Ctrl *ctrl, int t, unsigned int total)
+// Step number of a launch with the given slot; the first thread of the grid
+// hands t + 1 to the next launch (the other slot).  One store per launch: a
+// per-block completion count on one address serialises tens of thousands of
+// atomics per step at one L2 channel.
+__device__ __forceinline__ int begin_step(Ctrl *ctrl, int slot)
 {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned int prev = atomicAdd(&ctrl->done, 1u);
-        if (prev == total - 1) {
-            __hip_atomic_store(&ctrl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctrl->step, t + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    const int t = __hip_atomic_load(&ctrl->step[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+        __hip_atomic_store(&ctrl->step[slot ^ 1], t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return t;
 }
 
 // One tile (logical block bx of rows, shot chunk by) of step t.
-template <class T, int PREC, int SP, bool CPS, bool PACKED>
+// Variant V: 0 separate value arrays; 1 values packed per nonzero, gather loop
+// unrolled x2; 2 as 1 with streaming hints on data used once; 3 as 2, and the
+// row's own 20 gathered fields are kept from the diagonal nonzero instead of
+// being loaded again.
+template <class T, int PREC, int SP, bool CPS, int V>
 __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, int by)
 {
     constexpr bool EXACT = (PREC == 0);
     constexpr bool KAHAN = (PREC == 2);
+    constexpr bool PACKED = V >= 1;
+    constexpr bool NT = V >= 2;
+    constexpr bool CAP = V >= 3;
     constexpr int RPW = (SP >= 64) ? 1 : 64 / SP;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -84,6 +106,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         T mu1t = 0, mu1 = 0, mlx1 = 0, mlx2 = 0, mu2t = 0, mu2 = 0, mu3t = 0, mu3 = 0, mlx3 = 0, mlx4 = 0;
         T mw1t = 0, mw1 = 0, mly1 = 0, mly2 = 0, mw2t = 0, mw2 = 0, mw3t = 0, mw3 = 0, mly3 = 0, mly4 = 0;
         const int k0 = a.rp[row], k1 = a.rp[row + 1];
+        const size_t o = (size_t)row * S + s;
+        T cur[NFIELD];
+        bool got = false;
         auto nonzero = [&](int k) __attribute__((always_inline)) {
             const int j = a.ci[k];
             T am, a1, a2, a3, a4, a5, a6;
@@ -102,6 +127,16 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             const T w1t = x[W1T * F], w2t = x[W2T * F], w3t = x[W3T * F];
             const T lx1 = x[LX1 * F], lx2 = x[LX2 * F], lx3 = x[LX3 * F], lx4 = x[LX4 * F];
             const T ly1 = x[LY1 * F], ly2 = x[LY2 * F], ly3 = x[LY3 * F], ly4 = x[LY4 * F];
+            if constexpr (CAP) {
+                if (j == row) {
+                    cur[U1] = u1; cur[U2] = u2; cur[U3] = u3; cur[W1] = w1; cur[W2] = w2; cur[W3] = w3;
+                    cur[U1T] = u1t; cur[U2T] = u2t; cur[U3T] = u3t;
+                    cur[W1T] = w1t; cur[W2T] = w2t; cur[W3T] = w3t;
+                    cur[LX1] = lx1; cur[LX2] = lx2; cur[LX3] = lx3; cur[LX4] = lx4;
+                    cur[LY1] = ly1; cur[LY2] = ly2; cur[LY3] = ly3; cur[LY4] = ly4;
+                    got = true;
+                }
+            }
             // U_now / W_now as recombined at the end of the previous step (:775-776)
             const T U = u1 + u2 + u3;
             const T W = w1 + w2 + w3;
@@ -183,10 +218,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         const T rw7 = -dt * c33 * dyy * s6w - dt * dy * mly4 + mly4;
 
         // ---- own-row state ----------------------------------------------
-        const size_t o = (size_t)row * S + s;
-        T cur[NFIELD];
+        if (!CAP || !got) {
+            // (every row of an assembled operator holds its diagonal, so with
+            // CAP this runs only for a pattern without one)
 #pragma unroll
-        for (int f = 0; f < NFIELD; ++f) cur[f] = in[f * F + o];
+            for (int f = 0; f < NFIELD; ++f)
+                if (f < U1TT || f > W3TT) cur[f] = in[f * F + o];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) cur[U1TT + q] = ld_once<NT>(in + (U1TT + q) * F + o);
 
         // ---- lumped-mass solve with Dirichlet clamp (:681-729) ------------
         T an[6], ln[8];
@@ -231,19 +271,19 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                 const T *cin = a.comp + (size_t)(t & 1) * C6;
                 T *cout = a.comp + (size_t)((t + 1) & 1) * C6;
                 const T inc = v * dt + (ca * acc + al * an[q]) * dt * dt;
-                const T y = inc - cin[q * F + o];
+                const T y = inc - ld_once<NT>(cin + q * F + o);
                 const T sum = d + y;
-                cout[q * F + o] = (sum - d) - y;
+                st_once<NT>(cout + q * F + o, (sum - d) - y);
                 dn[q] = sum;
             } else {
                 dn[q] = d + v * dt + (ca * acc + al * an[q]) * dt * dt;
             }
-            out[(U1T + q) * F + o] = v + (cd * acc + de * an[q]) * dt;
-            out[(U1TT + q) * F + o] = an[q];
-            out[(U1 + q) * F + o] = dn[q];
+            st_once<NT>(out + (U1T + q) * F + o, v + (cd * acc + de * an[q]) * dt);
+            st_once<NT>(out + (U1TT + q) * F + o, an[q]);
+            st_once<NT>(out + (U1 + q) * F + o, dn[q]);
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) out[(LX1 + q) * F + o] = ln[q];
+        for (int q = 0; q < 8; ++q) st_once<NT>(out + (LX1 + q) * F + o, ln[q]);
 
         // ---- receivers sitting on this node --------------------------------
         const int r0 = a.rec_ptr[row], r1 = a.rec_ptr[row + 1];
@@ -260,15 +300,14 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
 
 }
 
-template <class T, int PREC, int SP, bool CPS, bool PACKED>
-__global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a)
+template <class T, int PREC, int SP, bool CPS, int V>
+__global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a, int slot)
 {
-    const int t = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int t = begin_step(a.ctrl, slot);
     const unsigned int nb = gridDim.x * gridDim.y;
     unsigned int bid = blockIdx.y * gridDim.x + blockIdx.x;
     if (a.xcd_remap) bid = xcd_contiguous(bid, nb);
-    step_tile<T, PREC, SP, CPS, PACKED>(a, t, (int)(bid % gridDim.x), (int)(bid / gridDim.x));
-    finish_step(a.ctrl, t, nb);
+    step_tile<T, PREC, SP, CPS, V>(a, t, (int)(bid % gridDim.x), (int)(bid / gridDim.x));
 }
 
 // Several runs (own mesh, medium, shots and step counts) in one launch per
@@ -280,10 +319,10 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a)
 template <class T, int PREC>
 __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__restrict__ args,
                                                          const int *__restrict__ tile_start, int nruns,
-                                                         int xcd_remap)
+                                                         int xcd_remap, int slot)
 {
     Ctrl *ctrl = args[0].ctrl;
-    const int t = __hip_atomic_load(&ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int t = begin_step(ctrl, slot);
     const unsigned int nb = gridDim.x;
     unsigned int bid = blockIdx.x;
     if (xcd_remap) bid = xcd_contiguous(bid, nb);
@@ -292,38 +331,39 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
     const StepArgs<T> &a = args[r];
     const int local = (int)bid - tile_start[r];
     const int bx = local % a.nbx, by = local / a.nbx;
-    const bool pk = a.variant == 1;
+    // separate value arrays or packed values (the streaming-hint variants are
+    // run as packed here: one launch serves runs of every variant)
+    const bool pk = a.variant != 0;
     if (a.sp == 1) {
-        if (a.cps) { if (pk) step_tile<T, PREC, 1, true, true>(a, t, bx, by); else step_tile<T, PREC, 1, true, false>(a, t, bx, by); }
-        else { if (pk) step_tile<T, PREC, 1, false, true>(a, t, bx, by); else step_tile<T, PREC, 1, false, false>(a, t, bx, by); }
+        if (a.cps) { if (pk) step_tile<T, PREC, 1, true, 1>(a, t, bx, by); else step_tile<T, PREC, 1, true, 0>(a, t, bx, by); }
+        else { if (pk) step_tile<T, PREC, 1, false, 1>(a, t, bx, by); else step_tile<T, PREC, 1, false, 0>(a, t, bx, by); }
     } else {
-        if (a.cps) { if (pk) step_tile<T, PREC, 64, true, true>(a, t, bx, by); else step_tile<T, PREC, 64, true, false>(a, t, bx, by); }
-        else { if (pk) step_tile<T, PREC, 64, false, true>(a, t, bx, by); else step_tile<T, PREC, 64, false, false>(a, t, bx, by); }
+        if (a.cps) { if (pk) step_tile<T, PREC, 64, true, 1>(a, t, bx, by); else step_tile<T, PREC, 64, true, 0>(a, t, bx, by); }
+        else { if (pk) step_tile<T, PREC, 64, false, 1>(a, t, bx, by); else step_tile<T, PREC, 64, false, 0>(a, t, bx, by); }
     }
-    finish_step(ctrl, t, nb);
 }
 
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
-                             int block, int xcd_remap, hipStream_t st)
+                             int block, int xcd_remap, int slot, hipStream_t st)
 {
     if (precision == 0)
         hipLaunchKernelGGL((multi_step_kernel<double, 0>), dim3(ntiles), dim3(block), 0, st,
-                           (const StepArgs<double> *)args_dev, tile_start_dev, nruns, xcd_remap);
+                           (const StepArgs<double> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
     else if (precision == 1)
         hipLaunchKernelGGL((multi_step_kernel<float, 1>), dim3(ntiles), dim3(block), 0, st,
-                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap);
+                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
     else if (precision == 2)
         hipLaunchKernelGGL((multi_step_kernel<float, 2>), dim3(ntiles), dim3(block), 0, st,
-                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap);
+                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-template <class T, int PREC, bool PK>
-static hipError_t launch_step_v(int sp, bool cps, const StepArgs<T> &a, dim3 g, int block, hipStream_t st)
+template <class T, int PREC, int V>
+static hipError_t launch_step_v(int sp, bool cps, const StepArgs<T> &a, dim3 g, int block, int slot, hipStream_t st)
 {
-#define SFG_L(SPV, C) hipLaunchKernelGGL((step_kernel<T, PREC, SPV, C, PK>), g, dim3(block), 0, st, a)
+#define SFG_L(SPV, C) hipLaunchKernelGGL((step_kernel<T, PREC, SPV, C, V>), g, dim3(block), 0, st, a, slot)
     switch (sp) {
     case 1: if (cps) SFG_L(1, true); else SFG_L(1, false); break;
     case 4: if (cps) SFG_L(4, true); else SFG_L(4, false); break;
@@ -337,21 +377,26 @@ static hipError_t launch_step_v(int sp, bool cps, const StepArgs<T> &a, dim3 g, 
 }
 
 template <class T, int PREC>
-static hipError_t launch_step_t(int sp, bool cps, const void *args, dim3 g, int block, hipStream_t st)
+static hipError_t launch_step_t(int sp, bool cps, const void *args, dim3 g, int block, int slot, hipStream_t st)
 {
     const StepArgs<T> &a = *static_cast<const StepArgs<T> *>(args);
-    if (a.variant == 1) return launch_step_v<T, PREC, true>(sp, cps, a, g, block, st);
-    return launch_step_v<T, PREC, false>(sp, cps, a, g, block, st);
+    switch (a.variant) {
+    case 0: return launch_step_v<T, PREC, 0>(sp, cps, a, g, block, slot, st);
+    case 1: return launch_step_v<T, PREC, 1>(sp, cps, a, g, block, slot, st);
+    case 2: return launch_step_v<T, PREC, 2>(sp, cps, a, g, block, slot, st);
+    case 3: return launch_step_v<T, PREC, 3>(sp, cps, a, g, block, slot, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_step(int precision, int sp, bool cps, const void *args, int nbx, int nby,
-                       int block, hipStream_t st)
+                       int block, int slot, hipStream_t st)
 {
     const dim3 g(nbx, nby);
     switch (precision) {
-    case 0: return launch_step_t<double, 0>(sp, cps, args, g, block, st);
-    case 1: return launch_step_t<float, 1>(sp, cps, args, g, block, st);
-    case 2: return launch_step_t<float, 2>(sp, cps, args, g, block, st);
+    case 0: return launch_step_t<double, 0>(sp, cps, args, g, block, slot, st);
+    case 1: return launch_step_t<float, 1>(sp, cps, args, g, block, slot, st);
+    case 2: return launch_step_t<float, 2>(sp, cps, args, g, block, slot, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -365,7 +410,7 @@ __global__ __launch_bounds__(256) void guard_kernel(const T *state, int n, int S
                                                     int every, int nodes_per_block)
 {
     __shared__ double red_u[256], red_w[256];
-    const int t = __hip_atomic_load(&ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int t = executed_steps(ctrl);
     const int sg = S < 64 ? S : 64;                 // shots covered by one block
     const int rstep = blockDim.x / sg;
     const int sl = threadIdx.x % sg, rl = threadIdx.x / sg;
