@@ -64,7 +64,7 @@ def main():
     ap.add_argument("--no-multi", action="store_true", help="skip config 4 part 2 (16 models in the same launch)")
     ap.add_argument("--quick", action="store_true", help="small mesh/steps for profiling runs")
     ap.add_argument("--launch", default="",
-                    help="pin the batched launch shape 'shots_per_lane_group,block,variant,xcd_remap' (no tuning)")
+                    help="pin the batched launch 'shots_per_lane_group,block,variant,xcd_remap[,hilbert]' (no tuning)")
     ap.add_argument("--launch64", default="", help="pin the FP64 single-shot launch shape (same form)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -76,7 +76,7 @@ def main():
         args.nelem, args.steps, args.fp64_steps, args.cpu_seconds = 100, 200, 100, 2.0
         args.acc_shots = 1
 
-    from seisfem_amd import autotune, gpu, workloads
+    from seisfem_amd import autotune, gpu, host, workloads
 
     # measured copy bandwidth of this device (1 GiB buffers), reported beside the nominal peak
     copy_gbs, copy_ms = gpu.stream_copy(1 << 30, 20)
@@ -96,18 +96,25 @@ def main():
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
     base = dict(dt=case["dt"], f0=case["f0"], t0=case["t0"])
     # launch shape chosen by measurement on this problem (untimed)
+    coords = (prob["x"], prob["y"])
+    hilbert = host.locality_order(*coords)
+
+    def pinned(spec):
+        v = [int(q) for q in spec.split(",")]
+        return dict(shots_per_lane_group=v[0], block=v[1], kernel_variant=v[2], xcd_remap=bool(v[3]),
+                    row_order=hilbert if len(v) > 4 and v[4] else None)
+
     if args.launch:
-        sp, blk, var, xcd = (int(v) for v in args.launch.split(","))
-        best = dict(shots_per_lane_group=sp, block=blk, kernel_variant=var, xcd_remap=bool(xcd))
+        best = pinned(args.launch)
     else:
-        best, _ = autotune.tune(op, md, case["src"], case["rec"], precision=prec, steps=30, log=log, **base)
+        best, _ = autotune.tune(op, md, case["src"], case["rec"], precision=prec, steps=30, log=log, coords=coords,
+                                **base)
     if args.launch64:
-        sp, blk, var, xcd = (int(v) for v in args.launch64.split(","))
-        best64 = dict(shots_per_lane_group=sp, block=blk, kernel_variant=var, xcd_remap=bool(xcd))
+        best64 = pinned(args.launch64)
     else:
         best64, _ = autotune.tune(op, md, case["src"][:1], case["rec"], precision=gpu.FP64, steps=30, log=log,
-                                  **base)
-    log(f"tuned batched {best}; tuned fp64 single {best64}")
+                                  coords=coords, **base)
+    log(f"tuned batched {autotune.describe(best)}; tuned fp64 single {autotune.describe(best64)}")
     kw = dict(base, **best)
     kw64 = dict(base, **best64)
 
@@ -186,9 +193,10 @@ def main():
             mop = gpu.Operator(p["node_num"], p["csr"])
             mmd = gpu.Medium(mop, p["c"], p["mass_lump"], p["dirichlet"], p["mpml"])
             keep += [mop, mmd]
+            morder = None if kw.get("row_order") is None else host.locality_order(p["x"], p["y"])
             runs.append(gpu.Run(mop, mmd, c["src"], K, c["rec"], c["dt"], c["f0"], c["t0"], precision=prec,
                                 nsteps=c["nsteps"], block=kw.get("block", 0), shots_per_lane_group=1,
-                                kernel_variant=kw.get("kernel_variant", 0)))
+                                kernel_variant=kw.get("kernel_variant", 0), row_order=morder))
         grp = gpu.Multi(runs)
         grp.advance(K)
         ms_m = grp.last_ms_per_step
@@ -262,7 +270,8 @@ def main():
         "config": {"workload": f"{args.shots} shots x Q4 {args.nelem}x{args.nelem} (dx=10 m), "
                                f"uneven steps U[{int(nsteps.min())},{K}], {args.precision}",
                    "nodes": N, "dof_per_shot": 2 * N, "shots": args.shots, "precision": args.precision,
-                   "parallelism": "1 GPU", "tuned": best, "tuned_fp64": best64},
+                   "parallelism": "1 GPU", "tuned": autotune.describe(best),
+                   "tuned_fp64": autotune.describe(best64)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(f"{args.precision}_{args.shots}x_q4_{args.nelem}"),

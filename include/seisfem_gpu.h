@@ -142,6 +142,9 @@ typedef struct {
                                   arrays, 3 = 1 + streaming hints on data used once per step,
                                   4 = 3 + own-row fields kept from the diagonal gather.  Same results
                                   bit for bit; a multi-run launch runs 3 and 4 as 1 */
+    const int *row_order;      /* [node_num] permutation: the order in which the step kernel's
+                                  waves take rows (e.g. sfh_locality_order), NULL = node order.
+                                  Changes only which rows run together, not any result */
 } sfg_shot_desc;
 
 /* Creates a run with all state zeroed (elastic_wave.c:466-502). */

@@ -87,6 +87,13 @@ void sfh_dirichlet_flags(int node_num, const double *node_x, const double *node_
                          double edge_size, double xmin, double xmax, double ymin, double ymax,
                          int free_surface, int *flag);
 
+/* Locality order of the nodes: node ids sorted along a Hilbert curve over
+ * their coordinates (quantised to a 2^16 x 2^16 grid of square cells, ties by
+ * id).  Stepping rows in this order keeps the rows in flight on the GPU a
+ * compact patch whose neighbours stay in L2 (sfg_shot_desc.row_order).  No
+ * counterpart in the reference, which steps rows in node order. */
+int sfh_locality_order(int node_num, const double *x, const double *y, int *order);
+
 /* Union of several CSR patterns with the values of each matrix scattered
  * onto it (0.0 where a matrix has no entry).  up must hold node_num+1 ints;
  * uj/ux_m must hold max_nnz entries.  Returns the union nnz or a negative code. */

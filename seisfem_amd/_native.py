@@ -27,7 +27,7 @@ class ShotDesc(C.Structure):
         ("dt", _d), ("f0", _d), ("t0", _d), ("amplitude", _d), ("angle_deg", _d),
         ("block", _i), ("shots_per_lane_group", _i), ("guard_every", _i), ("use_graph", _i),
         ("xcd_remap", _i), ("src_count", _pi), ("src_nodes", _pi), ("src_weight", _pd),
-        ("kernel_variant", _i),
+        ("kernel_variant", _i), ("row_order", _pi),
     ]
 
 
@@ -56,6 +56,7 @@ def lib():
         "sfh_node_location": (_i, [_i, _d, _pd, _pd, _d, _d]),
         "sfh_station_line": (None, [_i, _d, _d, _d, _d, _pd, _pd]),
         "sfh_dirichlet_flags": (None, [_i, _pd, _pd, _d, _d, _d, _d, _d, _i, _pi]),
+        "sfh_locality_order": (_i, [_i, _pd, _pd, _pi]),
         "sfh_csr_union": (_i, [_i, _i, C.POINTER(_pi), C.POINTER(_pi), C.POINTER(_pd), _i, _pi, _pi,
                                C.POINTER(_pd)]),
         # GPU

@@ -22,17 +22,31 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def run_case(name, case, forms=("fp64", "fp32", "fp32_kahan"), sources=None):
+def run_case(name, case, forms=("fp64", "fp32", "fp32_kahan"), sources=None, dt_safety=0.8):
+    """The configured dt is checked against dt_max = 2/sqrt(lambda_max(M_L^-1 K))
+    (sfg_estimate_dt); a dt above 0.95 dt_max is replaced by dt_safety * dt_max
+    (rounded down to 1 us) with the step count scaled to the same simulated
+    time, and both are reported."""
     prob = case["problem"]
     N = prob["node_num"]
     steps = case["steps_exec"]
     op = gpu.Operator(N, prob["csr"])
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
-    out = {"config": name, "nodes": N, "dof": 2 * N, "steps": steps, "receivers": int(case["rec"].size)}
+    lam, dt_max, _ = gpu.estimate_dt(op, md)
+    dt = case["dt"]
+    out = {"config": name, "nodes": N, "dof": 2 * N, "receivers": int(case["rec"].size),
+           "dt_config": dt, "dt_max_estimate": dt_max}
+    if dt > 0.95 * dt_max:
+        dt_new = np.floor(dt_safety * dt_max * 1e6) / 1e6
+        steps = int(np.ceil(steps * dt / dt_new))
+        dt = dt_new
+        print(f"[accuracy] {name}: dt {case['dt']:g} above the stability limit {dt_max:.4g}; using {dt:g} "
+              f"x {steps} steps", file=sys.stderr, flush=True)
+    out["dt"], out["steps"] = dt, steps
     ref = None
     for f in forms:
         t0 = time.time()
-        run = gpu.Run(op, md, case["src"][:1], steps, case["rec"], case["dt"], case["f0"], case["t0"],
+        run = gpu.Run(op, md, case["src"][:1], steps, case["rec"], dt, case["f0"], case["t0"],
                       precision=gpu.PRECISION[f], sources=sources)
         run.advance(steps)
         u, w = run.seismograms()

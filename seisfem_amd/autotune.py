@@ -1,50 +1,67 @@
 """Autotuner for the step kernel: picks the block size, the lane grouping
-(rows per wavefront vs shots per lane group), the kernel variant and the XCD
-block remap by timing short runs of the actual problem on the GPU.
+(rows per wavefront vs shots per lane group), the kernel variant, the XCD
+block remap and the row order (node order or a Hilbert walk over the node
+coordinates) by timing short runs of the actual problem on the GPU.
 
-Each candidate runs ``steps`` steps (after ``warmup``) with the energy guard
-only at the end, timed with HIP events on the run's stream; the fastest
-candidate wins.  Results are cached per (nodes, nnz, shots, precision).
+Each candidate runs ``steps`` steps (after ``steps`` untimed ones) as one
+captured graph with the energy guard at the end, timed with HIP events on the
+run's stream; the fastest candidate wins.  Results are cached per (nodes, nnz,
+shots, precision).  Every candidate gives the same results bit for bit.
 """
 import itertools
 
-from . import gpu
+from . import gpu, host
 
 _cache = {}
 
 
 def candidates(nshots):
-    """(shots per lane group, block, kernel variant, xcd remap) to try."""
+    """(shots per lane group, block, kernel variant, xcd remap, row order) to try."""
     if nshots <= 1:
-        sps = [1]
+        sps, blocks = [1], [64, 128, 256]
     elif nshots <= 4:
-        sps = [4, 16]
+        sps, blocks = [4, 16], [128, 256]
     elif nshots <= 16:
-        sps = [16, 64]
+        sps, blocks = [16, 64], [128, 256]
     else:
-        sps = [16, 32, 64]
-    return list(itertools.product(sps, [64, 128, 256], [1, 2, 3, 4], [True, False]))
+        sps, blocks = [32, 64], [128, 256]
+    return list(itertools.product(sps, blocks, [1, 3, 4], [True, False], ["node", "hilbert"]))
 
 
-def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None):
-    """Returns (best dict, table of (candidate, ms/step))."""
+def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None, coords=None):
+    """Returns (best dict of gpu.Run keyword arguments, table of (candidate, ms/step)).
+    coords = (x, y) of the nodes enables the Hilbert row order candidates."""
     nshots = len(src)
-    key = (op.node_num, op.nnz, nshots, precision)
+    key = (op.node_num, op.nnz, nshots, precision, coords is not None)
     if key in _cache:
         return _cache[key]
+    orders = {"node": None}
+    if coords is not None:
+        orders["hilbert"] = host.locality_order(*coords)
     table = []
-    for sp, block, variant, xcd in (cands or candidates(nshots)):
+    for sp, block, variant, xcd, oname in (cands or candidates(nshots)):
+        if oname not in orders:
+            continue
         # two graph replays of `steps` launches: the first warms up, the second is timed
         run = gpu.Run(op, md, src, 2 * steps, rec, dt, f0, t0, precision=precision, block=block,
-                      shots_per_lane_group=sp, kernel_variant=variant, xcd_remap=xcd, guard_every=steps)
+                      shots_per_lane_group=sp, kernel_variant=variant, xcd_remap=xcd, guard_every=steps,
+                      row_order=orders[oname])
         run.advance(steps)
         run.advance(steps)
         ms = run.last_ms_per_step
         run.close()
-        cand = dict(shots_per_lane_group=sp, block=block, kernel_variant=variant, xcd_remap=xcd)
+        cand = dict(shots_per_lane_group=sp, block=block, kernel_variant=variant, xcd_remap=xcd, row_order=oname)
         table.append((cand, ms))
         if log:
             log(f"tune {cand}: {ms:.4f} ms/step")
-    best = min(table, key=lambda c: c[1])[0]
+    best = dict(min(table, key=lambda c: c[1])[0])
+    best["row_order"] = orders[best["row_order"]]
     _cache[key] = (best, table)
     return best, table
+
+
+def describe(best):
+    """JSON-friendly form of a tuned launch (the row order by name)."""
+    d = dict(best)
+    d["row_order"] = "node" if d.get("row_order") is None else "hilbert"
+    return d

@@ -386,6 +386,7 @@ struct sfg_run {
     void *state = nullptr, *comp = nullptr;
     Ctrl *ctrl = nullptr;
     int *active = nullptr, *src = nullptr, *rec_ptr = nullptr, *rec_idx = nullptr;
+    int *order = nullptr;       // row_order on the device (or null)
     int *src_shot = nullptr;    // src = per-node entry pointers, then shot and weights per entry
     double *src_wt = nullptr;
     double *wavelet = nullptr, *seis_u = nullptr, *seis_w = nullptr, *energy = nullptr;
@@ -410,6 +411,7 @@ static void run_free(sfg_run *r)
     dfree(r->comp);
     dfree(r->ctrl);
     dfree(r->active);
+    dfree(r->order);
     dfree(r->src);
     dfree(r->src_shot);
     dfree(r->src_wt);
@@ -602,6 +604,17 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         for (int q = 0; q < d->rec_num; ++q) ridx[fillp[d->rec_node[q]]++] = q;
     }
     r->nslots = (d->max_steps + r->guard_every - 1) / r->guard_every + 1;
+    if (d->row_order) {
+        std::vector<char> seen(n, 0);
+        for (int i = 0; i < n; ++i) {
+            const int v = d->row_order[i];
+            if (v < 0 || v >= n || seen[v]) {
+                run_free(r);
+                return fail(SFG_E_ARG, "run: row_order is not a permutation of the %d nodes", n);
+            }
+            seen[v] = 1;
+        }
+    }
 
     const size_t F = (size_t)n * r->S;
     const size_t w = r->prec == 0 ? 8 : 4;
@@ -614,7 +627,8 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         (rc = dalloc(&r->src_shot, sshot.size())) || (rc = dalloc(&r->src_wt, swt.size())) ||
         (rc = dalloc(&r->rec_ptr, n + 1)) || (rc = dalloc(&r->rec_idx, ridx.size())) ||
         (rc = dalloc(&r->wavelet, d->max_steps)) || (rc = dalloc(&r->seis_u, seis_n)) ||
-        (rc = dalloc(&r->seis_w, seis_n)) || (rc = dalloc(&r->energy, (size_t)r->nslots * r->S * 2))) {
+        (rc = dalloc(&r->seis_w, seis_n)) || (rc = dalloc(&r->energy, (size_t)r->nslots * r->S * 2)) ||
+        (d->row_order && (rc = dalloc(&r->order, n)))) {
         run_free(r);
         return rc;
     }
@@ -628,6 +642,8 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (e == hipSuccess) e = launch_fill_zero(r->seis_w, seis_n * 8, r->stream);
     if (e == hipSuccess) e = launch_fill_zero(r->energy, (size_t)r->nslots * r->S * 16, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->active, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess && r->order)
+        e = hipMemcpyAsync(r->order, d->row_order, sizeof(int) * n, hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->src, sptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->src_shot, sshot.data(), sizeof(int) * sshot.size(), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->src_wt, swt.data(), sizeof(double) * swt.size(), hipMemcpyHostToDevice, r->stream);
@@ -680,6 +696,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.cps = r->cps ? 1 : 0;
         a.nbx = r->nbx;
         a.rp = op->rp;
+        a.order = r->order;
         a.ci = op->ci;
         for (int m = 0; m < 7; ++m) a.A[m] = vals + (size_t)m * op->nnz;
         // descriptor variant -> kernel variant (see step_tile)

@@ -54,6 +54,7 @@ struct StepArgs {
     int xcd_remap;          // 1: contiguous block ranges per XCD
     int sp, cps, nbx;       // lane grouping, per-shot coefficients, row blocks (multi-run dispatch)
     const int *rp, *ci;     // union CSR pattern
+    const int *order;       // row taken by logical row i (a permutation), or null = i
     const T *A[7];          // M, S1..S6 on the union pattern (one array each)
     const T *Ap;            // the same values packed per nonzero: [nnz][8] = M, S1..S6, 0
     int variant;            // 0: separate value arrays; 1: packed values, gather loop unrolled x2

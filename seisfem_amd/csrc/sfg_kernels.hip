@@ -93,8 +93,14 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
     if constexpr (SP >= 64) row = __builtin_amdgcn_readfirstlane(row);
     const int s = by * (SP >= 64 ? 64 : SP) + (SP >= 64 ? lane : lane % SP);
     const int nact = (t < a.nsteps_max) ? a.active[t] : 0;
+    if (row >= a.n) return;
+    if (a.order) {
+        // rows in locality order: the waves in flight hold a compact patch of the mesh
+        row = a.order[row];
+        if constexpr (SP >= 64) row = __builtin_amdgcn_readfirstlane(row);
+    }
 
-    if (row < a.n && s < nact) {
+    if (s < nact) {
         const size_t F = a.fstride;
         const int S = a.S;
         const T *in = a.state + (size_t)(t & 1) * NFIELD * F;

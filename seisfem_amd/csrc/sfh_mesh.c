@@ -10,7 +10,9 @@
  * Build with -ffp-contract=off: coordinates must round exactly as the reference.
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include "seisfem_host.h"
 
 int sfh_element_order(int type_code)
@@ -156,4 +158,70 @@ void sfh_dirichlet_flags(int node_num, const double *X, const double *Y, double 
         if (!free_surface && X[i] >= xmax - q) on_side = 1;
         flag[i] = on_side ? 1 : 0;
     }
+}
+
+/* ---- locality order of the nodes (no counterpart in the reference) -------- */
+
+/* Index of cell (x, y) along the Hilbert curve that fills a 2^16 x 2^16 grid:
+ * at each level from the top, the quadrant gives two bits of the index and
+ * the coordinates are reflected / swapped into the quadrant's frame. */
+static uint64_t hilbert_index(uint32_t x, uint32_t y)
+{
+    uint64_t d = 0;
+    for (uint32_t s = 1u << 15; s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u;
+        const uint32_t ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = (s - 1) - (x & (s - 1));
+                y = (s - 1) - (y & (s - 1));
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+        x &= s - 1;
+        y &= s - 1;
+    }
+    return d;
+}
+
+typedef struct {
+    uint64_t key;
+    int node;
+} sfh_key_node;
+
+static int cmp_key_node(const void *a, const void *b)
+{
+    const sfh_key_node *p = (const sfh_key_node *)a, *q = (const sfh_key_node *)b;
+    if (p->key != q->key) return p->key < q->key ? -1 : 1;
+    return (p->node > q->node) - (p->node < q->node);
+}
+
+int sfh_locality_order(int node_num, const double *x, const double *y, int *order)
+{
+    if (node_num <= 0 || !x || !y || !order) return SFH_E_ARG;
+    double x0 = x[0], x1 = x[0], y0 = y[0], y1 = y[0];
+    for (int i = 1; i < node_num; ++i) {
+        if (x[i] < x0) x0 = x[i];
+        if (x[i] > x1) x1 = x[i];
+        if (y[i] < y0) y0 = y[i];
+        if (y[i] > y1) y1 = y[i];
+    }
+    /* one scale for both axes keeps the cells square */
+    const double span = fmax(x1 - x0, y1 - y0);
+    const double sc = span > 0.0 ? 65535.0 / span : 0.0;
+    sfh_key_node *kn = (sfh_key_node *)malloc(sizeof(sfh_key_node) * (size_t)node_num);
+    if (!kn) return SFH_E_NOMEM;
+    for (int i = 0; i < node_num; ++i) {
+        const uint32_t cx = (uint32_t)fmin(65535.0, fmax(0.0, (x[i] - x0) * sc + 0.5));
+        const uint32_t cy = (uint32_t)fmin(65535.0, fmax(0.0, (y[i] - y0) * sc + 0.5));
+        kn[i].key = hilbert_index(cx, cy);
+        kn[i].node = i;
+    }
+    qsort(kn, (size_t)node_num, sizeof(sfh_key_node), cmp_key_node);
+    for (int i = 0; i < node_num; ++i) order[i] = kn[i].node;
+    free(kn);
+    return SFH_OK;
 }

@@ -88,9 +88,11 @@ class Medium:
 class Run:
     def __init__(self, op, md, src_nodes, max_steps, rec_nodes, dt, f0, t0, amplitude=1.0e10,
                  angle_deg=90.0, precision=FP64, nsteps=None, block=0, shots_per_lane_group=0,
-                 guard_every=0, use_graph=True, xcd_remap=True, sources=None, kernel_variant=0):
+                 guard_every=0, use_graph=True, xcd_remap=True, sources=None, kernel_variant=0, row_order=None):
         """sources: optional per-shot list of (nodes, weights[k][2]) distributed
-        sources (u and w weights on each node) replacing the point forces."""
+        sources (u and w weights on each node) replacing the point forces.
+        row_order: optional permutation of the nodes giving the order in which
+        rows are stepped (host.locality_order); results do not depend on it."""
         L = lib()
         self.op, self.md = op, md
         self.src = np.ascontiguousarray(np.atleast_1d(src_nodes), np.int32)
@@ -111,6 +113,9 @@ class Run:
         d.guard_every, d.use_graph = guard_every, int(bool(use_graph))
         d.xcd_remap = 0 if xcd_remap else -1
         d.kernel_variant = int(kernel_variant)
+        if row_order is not None:
+            self._order = np.ascontiguousarray(row_order, np.int32)
+            d.row_order = _pi(self._order)
         if sources is not None:
             if len(sources) != self.nshots:
                 raise ValueError("need one source description per shot")

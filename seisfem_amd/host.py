@@ -100,6 +100,17 @@ def station_line(num, x_first, y_first, x_last, y_last):
     return xs, ys
 
 
+def locality_order(x, y):
+    """Node ids along a Hilbert curve over the coordinates (sfh_locality_order)."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    order = np.empty(x.size, np.int32)
+    rc = lib().sfh_locality_order(x.size, _pd(x), _pd(y), _pi(order))
+    if rc != 0:
+        raise ValueError(f"sfh_locality_order failed with code {rc}")
+    return order
+
+
 def dirichlet_flags(x, y, edge_size, xmin, xmax, ymin, ymax, free_surface):
     f = np.empty(x.size, np.int32)
     lib().sfh_dirichlet_flags(x.size, _pd(x), _pd(y), edge_size, xmin, xmax, ymin, ymax,

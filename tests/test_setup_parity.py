@@ -173,3 +173,18 @@ def test_total_mass_and_partition_of_unity():
             return d
         assert np.allclose(dense(1), dense(1).T, atol=1e-12)
         assert np.allclose(dense(3), dense(4).T, atol=1e-12)
+
+
+def test_locality_order_is_a_hilbert_walk():
+    """On a 64 x 64 lattice every cell gets its own level-6 Hilbert cell, so
+    consecutive nodes of the order are lattice neighbours (a Z/Morton order or
+    a row-major order would jump); on an irregular cloud it is a permutation."""
+    x, y = np.meshgrid(np.arange(64.0), np.arange(64.0), indexing="ij")
+    x, y = x.ravel(), y.ravel()
+    o = host.locality_order(x, y)
+    assert np.array_equal(np.sort(o), np.arange(x.size))
+    assert np.all(np.abs(np.diff(x[o])) + np.abs(np.diff(y[o])) == 1.0)
+    rng = np.random.default_rng(3)
+    xr, yr = rng.uniform(-5.0, 7.0, 1000), rng.uniform(100.0, 101.0, 1000)
+    o = host.locality_order(xr, yr)
+    assert np.array_equal(np.sort(o), np.arange(1000))
