@@ -441,12 +441,23 @@ static int launch_one_step(sfg_run *r, int slot)
     return SFG_OK;
 }
 
+// Layout of the run's 26-field state (the step kernel derives the same from SP).
+static StateLayout state_layout(const sfg_run *r)
+{
+    StateLayout L;
+    L.gs = node_major(r->SP) ? (r->SP >= 64 ? 64 : r->SP) : 0;
+    L.ng = L.gs ? r->S / L.gs : 0;
+    L.nf = NFIELD;
+    L.S = r->S;
+    L.F = (size_t)r->op->n * r->S;
+    return L;
+}
+
 // ctrl / stream default to the run's own (a multi-run passes its shared ones)
 static int launch_guard_now(sfg_run *r, const Ctrl *ctrl = nullptr, hipStream_t st = nullptr)
 {
-    const size_t F = (size_t)r->op->n * r->S;
-    HIPCHK(launch_guard(r->prec, r->state, r->op->n, r->S, F, ctrl ? ctrl : r->ctrl, r->energy, r->nslots,
-                        r->guard_every, st ? st : r->stream));
+    HIPCHK(launch_guard(r->prec, r->state, r->op->n, state_layout(r), ctrl ? ctrl : r->ctrl, r->energy,
+                        r->nslots, r->guard_every, st ? st : r->stream));
     return SFG_OK;
 }
 
@@ -868,7 +879,7 @@ extern "C" int sfg_run_snapshot(sfg_run *r, int shot, double *U, double *W)
     int rc = dalloc(&dU, 2 * (size_t)n);
     if (rc) return rc;
     dW = dU + n;
-    hipError_t e = launch_gather_uw(r->prec, r->state, n, r->S, (size_t)n * r->S, s, done & 1, dU, dW, r->stream);
+    hipError_t e = launch_gather_uw(r->prec, r->state, n, state_layout(r), s, done & 1, dU, dW, r->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
     if (e == hipSuccess) e = hipMemcpy(U, dU, sizeof(double) * n, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(W, dW, sizeof(double) * n, hipMemcpyDeviceToHost);

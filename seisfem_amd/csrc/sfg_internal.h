@@ -45,6 +45,29 @@ __device__ __forceinline__ int executed_steps(const Ctrl *c)
     return a > b ? a : b;
 }
 
+// Where value (node i, field f, shot s) of a state array lives.  Field-major,
+// [field][node][shot], for lane groups of 1 or 4 shots (a wave reads one field
+// of 64 or 16 consecutive rows); node-major, [node][shot group][field][shot in
+// group], for groups of 16, 32 or 64 shots, so that a wave's gather of one
+// neighbour reads all that node's fields from one contiguous block instead of
+// one short piece from each of 20 arrays tens of MB apart.
+struct StateLayout {
+    int gs;         // shots per group when node-major, 0 = field-major
+    int ng;         // shot groups (node-major)
+    int nf;         // fields per node in this array (26 state, 6 Kahan terms)
+    int S;          // padded shot count
+    size_t F;       // n * S: field stride of the field-major form
+};
+
+__host__ __device__ inline size_t state_index(const StateLayout &L, int i, int f, int s)
+{
+    if (L.gs) return (((size_t)i * L.ng + s / L.gs) * L.nf + f) * L.gs + s % L.gs;
+    return (size_t)f * L.F + (size_t)i * L.S + s;
+}
+
+// Lane groups of SP shots use the node-major layout from 16 shots up.
+constexpr bool node_major(int sp) { return sp >= 16; }
+
 template <class T>
 struct StepArgs {
     int n;                  // nodes (rows)
@@ -83,9 +106,9 @@ hipError_t launch_step(int precision, int sp, bool coef_per_shot, const void *ar
                        int nblk_y, int block, int slot, hipStream_t st);
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
                              int block, int xcd_remap, int slot, hipStream_t st);
-hipError_t launch_guard(int precision, const void *state, int n, int S, size_t fstride,
+hipError_t launch_guard(int precision, const void *state, int n, const StateLayout &L,
                         const Ctrl *ctrl, double *energy, int nslots, int every, hipStream_t st);
-hipError_t launch_gather_uw(int precision, const void *state, int n, int S, size_t fstride,
+hipError_t launch_gather_uw(int precision, const void *state, int n, const StateLayout &L,
                             int shot, int parity, double *U, double *W, hipStream_t st);
 hipError_t launch_spmv64(int n, const int *rp, const int *cj, const double *a, const double *x,
                          double *y, hipStream_t st);

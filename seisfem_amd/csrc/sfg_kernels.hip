@@ -91,7 +91,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
     const int wpb = blockDim.x >> 6;
     int row = (bx * wpb + wave) * RPW + (SP >= 64 ? 0 : lane / SP);
     if constexpr (SP >= 64) row = __builtin_amdgcn_readfirstlane(row);
-    const int s = by * (SP >= 64 ? 64 : SP) + (SP >= 64 ? lane : lane % SP);
+    constexpr int GS = SP >= 64 ? 64 : SP;      // shots of one lane group
+    const int ls = SP >= 64 ? lane : lane % SP;  // this lane's shot within the group
+    const int s = by * GS + ls;
     const int nact = (t < a.nsteps_max) ? a.active[t] : 0;
     if (row >= a.n) return;
     if (a.order) {
@@ -105,6 +107,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         const int S = a.S;
         const T *in = a.state + (size_t)(t & 1) * NFIELD * F;
         T *out = a.state + (size_t)((t + 1) & 1) * NFIELD * F;
+        // element (node, field f) of this lane's shot is at node_base(node) + f * FS
+        constexpr bool NM = node_major(SP);
+        const size_t FS = NM ? (size_t)GS : F;
+        auto node_base = [&](int node) __attribute__((always_inline)) -> size_t {
+            if constexpr (NM) return ((size_t)node * (S / GS) + by) * (NFIELD * GS) + ls;
+            else return (size_t)node * S + s;
+        };
 
         // ---- the 32 distinct products over one walk of the row ------------
         T s1u = 0, s1w = 0, s2u = 0, s2w = 0, s3u = 0, s3w = 0;
@@ -112,7 +121,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         T mu1t = 0, mu1 = 0, mlx1 = 0, mlx2 = 0, mu2t = 0, mu2 = 0, mu3t = 0, mu3 = 0, mlx3 = 0, mlx4 = 0;
         T mw1t = 0, mw1 = 0, mly1 = 0, mly2 = 0, mw2t = 0, mw2 = 0, mw3t = 0, mw3 = 0, mly3 = 0, mly4 = 0;
         const int k0 = a.rp[row], k1 = a.rp[row + 1];
-        const size_t o = (size_t)row * S + s;
+        const size_t o = node_base(row);
         T cur[NFIELD];
         bool got = false;
         auto nonzero = [&](int k) __attribute__((always_inline)) {
@@ -126,13 +135,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                 a1 = a.A[1][k]; a2 = a.A[2][k]; a3 = a.A[3][k];
                 a4 = a.A[4][k]; a5 = a.A[5][k]; a6 = a.A[6][k];
             }
-            const T *x = in + (size_t)j * S + s;
-            const T u1 = x[U1 * F], u2 = x[U2 * F], u3 = x[U3 * F];
-            const T w1 = x[W1 * F], w2 = x[W2 * F], w3 = x[W3 * F];
-            const T u1t = x[U1T * F], u2t = x[U2T * F], u3t = x[U3T * F];
-            const T w1t = x[W1T * F], w2t = x[W2T * F], w3t = x[W3T * F];
-            const T lx1 = x[LX1 * F], lx2 = x[LX2 * F], lx3 = x[LX3 * F], lx4 = x[LX4 * F];
-            const T ly1 = x[LY1 * F], ly2 = x[LY2 * F], ly3 = x[LY3 * F], ly4 = x[LY4 * F];
+            const T *x = in + node_base(j);
+            const T u1 = x[U1 * FS], u2 = x[U2 * FS], u3 = x[U3 * FS];
+            const T w1 = x[W1 * FS], w2 = x[W2 * FS], w3 = x[W3 * FS];
+            const T u1t = x[U1T * FS], u2t = x[U2T * FS], u3t = x[U3T * FS];
+            const T w1t = x[W1T * FS], w2t = x[W2T * FS], w3t = x[W3T * FS];
+            const T lx1 = x[LX1 * FS], lx2 = x[LX2 * FS], lx3 = x[LX3 * FS], lx4 = x[LX4 * FS];
+            const T ly1 = x[LY1 * FS], ly2 = x[LY2 * FS], ly3 = x[LY3 * FS], ly4 = x[LY4 * FS];
             if constexpr (CAP) {
                 if (j == row) {
                     cur[U1] = u1; cur[U2] = u2; cur[U3] = u3; cur[W1] = w1; cur[W2] = w2; cur[W3] = w3;
@@ -229,10 +238,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             // CAP this runs only for a pattern without one)
 #pragma unroll
             for (int f = 0; f < NFIELD; ++f)
-                if (f < U1TT || f > W3TT) cur[f] = in[f * F + o];
+                if (f < U1TT || f > W3TT) cur[f] = in[f * FS + o];
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) cur[U1TT + q] = ld_once<NT>(in + (U1TT + q) * F + o);
+        for (int q = 0; q < 6; ++q) cur[U1TT + q] = ld_once<NT>(in + (U1TT + q) * FS + o);
 
         // ---- lumped-mass solve with Dirichlet clamp (:681-729) ------------
         T an[6], ln[8];
@@ -273,23 +282,27 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         for (int q = 0; q < 6; ++q) {
             const T d = cur[U1 + q], v = cur[U1T + q], acc = cur[U1TT + q];
             if constexpr (KAHAN) {
+                // Kahan terms: 6 fields in the same layout as the state
                 const size_t C6 = (size_t)6 * F;
                 const T *cin = a.comp + (size_t)(t & 1) * C6;
                 T *cout = a.comp + (size_t)((t + 1) & 1) * C6;
+                size_t co;
+                if constexpr (NM) co = ((size_t)row * (S / GS) + by) * (6 * GS) + ls;
+                else co = o;
                 const T inc = v * dt + (ca * acc + al * an[q]) * dt * dt;
-                const T y = inc - ld_once<NT>(cin + q * F + o);
+                const T y = inc - ld_once<NT>(cin + q * FS + co);
                 const T sum = d + y;
-                st_once<NT>(cout + q * F + o, (sum - d) - y);
+                st_once<NT>(cout + q * FS + co, (sum - d) - y);
                 dn[q] = sum;
             } else {
                 dn[q] = d + v * dt + (ca * acc + al * an[q]) * dt * dt;
             }
-            st_once<NT>(out + (U1T + q) * F + o, v + (cd * acc + de * an[q]) * dt);
-            st_once<NT>(out + (U1TT + q) * F + o, an[q]);
-            st_once<NT>(out + (U1 + q) * F + o, dn[q]);
+            st_once<NT>(out + (U1T + q) * FS + o, v + (cd * acc + de * an[q]) * dt);
+            st_once<NT>(out + (U1TT + q) * FS + o, an[q]);
+            st_once<NT>(out + (U1 + q) * FS + o, dn[q]);
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) st_once<NT>(out + (LX1 + q) * F + o, ln[q]);
+        for (int q = 0; q < 8; ++q) st_once<NT>(out + (LX1 + q) * FS + o, ln[q]);
 
         // ---- receivers sitting on this node --------------------------------
         const int r0 = a.rec_ptr[row], r1 = a.rec_ptr[row + 1];
@@ -411,25 +424,27 @@ hipError_t launch_step(int precision, int sp, bool cps, const void *args, int nb
 // Threads are laid out [rows][shots] (shots innermost, as the state), each
 // accumulates a strided set of nodes, then a block reduction per shot.
 template <class T>
-__global__ __launch_bounds__(256) void guard_kernel(const T *state, int n, int S, size_t F,
+__global__ __launch_bounds__(256) void guard_kernel(const T *state, int n, const StateLayout L,
                                                     const Ctrl *ctrl, double *energy, int nslots,
                                                     int every, int nodes_per_block)
 {
     __shared__ double red_u[256], red_w[256];
     const int t = executed_steps(ctrl);
+    const int S = L.S;
     const int sg = S < 64 ? S : 64;                 // shots covered by one block
     const int rstep = blockDim.x / sg;
     const int sl = threadIdx.x % sg, rl = threadIdx.x / sg;
     const int s = blockIdx.y * sg + sl;
-    const T *in = state + (size_t)(t & 1) * NFIELD * F;
+    const T *in = state + (size_t)(t & 1) * NFIELD * L.F;
     double eu = 0.0, ew = 0.0;
     const int n0 = blockIdx.x * nodes_per_block;
     const int n1 = min(n, n0 + nodes_per_block);
-    if (rl < rstep) {
+    if (rl < rstep && s < S) {
         for (int i = n0 + rl; i < n1; i += rstep) {
-            const size_t o = (size_t)i * S + s;
-            const double U = (double)(in[U1 * F + o] + in[U2 * F + o] + in[U3 * F + o]);
-            const double W = (double)(in[W1 * F + o] + in[W2 * F + o] + in[W3 * F + o]);
+            const double U = (double)(in[state_index(L, i, U1, s)] + in[state_index(L, i, U2, s)] +
+                                      in[state_index(L, i, U3, s)]);
+            const double W = (double)(in[state_index(L, i, W1, s)] + in[state_index(L, i, W2, s)] +
+                                      in[state_index(L, i, W3, s)]);
             eu += U * U;
             ew += W * W;
         }
@@ -437,7 +452,7 @@ __global__ __launch_bounds__(256) void guard_kernel(const T *state, int n, int S
     red_u[threadIdx.x] = eu;
     red_w[threadIdx.x] = ew;
     __syncthreads();
-    if (rl == 0) {
+    if (rl == 0 && s < S) {
         for (int q = 1; q < rstep; ++q) {
             eu += red_u[q * sg + sl];
             ew += red_w[q * sg + sl];
@@ -449,43 +464,45 @@ __global__ __launch_bounds__(256) void guard_kernel(const T *state, int n, int S
     }
 }
 
-hipError_t launch_guard(int precision, const void *state, int n, int S, size_t F, const Ctrl *ctrl,
+hipError_t launch_guard(int precision, const void *state, int n, const StateLayout &L, const Ctrl *ctrl,
                         double *energy, int nslots, int every, hipStream_t st)
 {
     const int npb = 2048;
-    const dim3 g((n + npb - 1) / npb, S < 64 ? 1 : S / 64);
+    const int sg = L.S < 64 ? L.S : 64;
+    const dim3 g((n + npb - 1) / npb, (L.S + sg - 1) / sg);
     if (precision == 0)
-        hipLaunchKernelGGL(guard_kernel<double>, g, dim3(256), 0, st, (const double *)state, n, S, F,
+        hipLaunchKernelGGL(guard_kernel<double>, g, dim3(256), 0, st, (const double *)state, n, L,
                            ctrl, energy, nslots, every, npb);
     else
-        hipLaunchKernelGGL(guard_kernel<float>, g, dim3(256), 0, st, (const float *)state, n, S, F,
+        hipLaunchKernelGGL(guard_kernel<float>, g, dim3(256), 0, st, (const float *)state, n, L,
                            ctrl, energy, nslots, every, npb);
     return hipGetLastError();
 }
 
 // ---- U = U1+U2+U3, W = W1+W2+W3 of one shot (snapshots, :775-776, :797-806) --
 template <class T>
-__global__ void gather_uw_kernel(const T *state, int n, int S, size_t F, int shot, int parity,
+__global__ void gather_uw_kernel(const T *state, int n, const StateLayout L, int shot, int parity,
                                  double *U, double *W)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const T *in = state + (size_t)parity * NFIELD * F;
-    const size_t o = (size_t)i * S + shot;
-    U[i] = (double)(in[U1 * F + o] + in[U2 * F + o] + in[U3 * F + o]);
-    W[i] = (double)(in[W1 * F + o] + in[W2 * F + o] + in[W3 * F + o]);
+    const T *in = state + (size_t)parity * NFIELD * L.F;
+    U[i] = (double)(in[state_index(L, i, U1, shot)] + in[state_index(L, i, U2, shot)] +
+                    in[state_index(L, i, U3, shot)]);
+    W[i] = (double)(in[state_index(L, i, W1, shot)] + in[state_index(L, i, W2, shot)] +
+                    in[state_index(L, i, W3, shot)]);
 }
 
-hipError_t launch_gather_uw(int precision, const void *state, int n, int S, size_t F, int shot,
+hipError_t launch_gather_uw(int precision, const void *state, int n, const StateLayout &L, int shot,
                             int parity, double *U, double *W, hipStream_t st)
 {
     const dim3 g((n + 255) / 256);
     if (precision == 0)
-        hipLaunchKernelGGL(gather_uw_kernel<double>, g, dim3(256), 0, st, (const double *)state, n, S,
-                           F, shot, parity, U, W);
+        hipLaunchKernelGGL(gather_uw_kernel<double>, g, dim3(256), 0, st, (const double *)state, n, L,
+                           shot, parity, U, W);
     else
-        hipLaunchKernelGGL(gather_uw_kernel<float>, g, dim3(256), 0, st, (const float *)state, n, S,
-                           F, shot, parity, U, W);
+        hipLaunchKernelGGL(gather_uw_kernel<float>, g, dim3(256), 0, st, (const float *)state, n, L,
+                           shot, parity, U, W);
     return hipGetLastError();
 }
 
