@@ -140,8 +140,10 @@ typedef struct {
     const double *src_weight;  /* [sum of src_count][2]: u and w weights */
     int kernel_variant;        /* 0 = default (1), 1 = packed values + 2-way gather, 2 = separate value
                                   arrays, 3 = 1 + streaming hints on data used once per step,
-                                  4 = 3 + own-row fields kept from the diagonal gather.  Same results
-                                  bit for bit; a multi-run launch runs 3 and 4 as 1 */
+                                  4 = 3 + own-row fields kept from the diagonal gather and own-row
+                                  accelerations / Kahan terms loaded first, 5 and 6 = 4 with 3 and 4
+                                  nonzeros' gathers in flight per trip.  Same results bit for bit;
+                                  a multi-run launch runs 3 to 6 as 1 */
     const int *row_order;      /* [node_num] permutation: the order in which the step kernel's
                                   waves take rows (e.g. sfh_locality_order), NULL = node order.
                                   Changes only which rows run together, not any result */

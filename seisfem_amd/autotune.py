@@ -18,14 +18,14 @@ _cache = {}
 def candidates(nshots):
     """(shots per lane group, block, kernel variant, xcd remap, row order) to try."""
     if nshots <= 1:
-        sps, blocks = [1], [64, 128, 256]
+        sps, blocks = [1], [64, 128]
     elif nshots <= 4:
-        sps, blocks = [4, 16], [128, 256]
+        sps, blocks = [4, 16], [256]
     elif nshots <= 16:
-        sps, blocks = [16, 64], [128, 256]
+        sps, blocks = [16, 64], [256]
     else:
-        sps, blocks = [32, 64], [128, 256]
-    return list(itertools.product(sps, blocks, [1, 3, 4], [True, False], ["node", "hilbert"]))
+        sps, blocks = [32, 64], [256]
+    return list(itertools.product(sps, blocks, [1, 3, 4, 5, 6], [True, False], ["node", "hilbert"]))
 
 
 def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None, coords=None):

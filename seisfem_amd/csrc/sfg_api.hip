@@ -490,7 +490,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (!out || !op || !media || nmedia <= 0 || !d) return fail(SFG_E_ARG, "run: bad arguments");
     *out = nullptr;
     if (d->precision < 0 || d->precision > 2) return fail(SFG_E_ARG, "run: unknown precision %d", d->precision);
-    if (d->kernel_variant < 0 || d->kernel_variant > 4)
+    if (d->kernel_variant < 0 || d->kernel_variant > 6)
         return fail(SFG_E_ARG, "run: unknown kernel_variant %d", d->kernel_variant);
     if (d->nshots <= 0 || !d->src_node) return fail(SFG_E_ARG, "run: need >= 1 shot with a source node");
     if (nmedia != 1 && nmedia != d->nshots) return fail(SFG_E_ARG, "run: need one medium or one per shot");
@@ -711,7 +711,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.ci = op->ci;
         for (int m = 0; m < 7; ++m) a.A[m] = vals + (size_t)m * op->nnz;
         // descriptor variant -> kernel variant (see step_tile)
-        static const int kv[5] = {1, 1, 0, 2, 3};
+        static const int kv[7] = {1, 1, 0, 2, 3, 4, 5};
         a.variant = kv[d->kernel_variant];
         a.coef = r->cps ? (decltype(coef))r->coefp : coef;
         a.flag = r->cps ? r->flagp : md->flag;

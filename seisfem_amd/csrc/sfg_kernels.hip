@@ -76,7 +76,8 @@ __device__ __forceinline__ int begin_step(Ctrl *ctrl, int slot)
 // Variant V: 0 separate value arrays; 1 values packed per nonzero, gather loop
 // unrolled x2; 2 as 1 with streaming hints on data used once; 3 as 2, and the
 // row's own 20 gathered fields are kept from the diagonal nonzero instead of
-// being loaded again.
+// being loaded again, and its accelerations and Kahan terms are loaded before
+// the gathers; 4 and 5 as 3 with 3 and 4 nonzeros' gathers in flight per trip.
 template <class T, int PREC, int SP, bool CPS, int V>
 __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, int by)
 {
@@ -85,6 +86,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
     constexpr bool PACKED = V >= 1;
     constexpr bool NT = V >= 2;
     constexpr bool CAP = V >= 3;
+    constexpr int UN = V >= 5 ? 4 : V == 4 ? 3 : 2;  // nonzeros per trip of the gather loop
     constexpr int RPW = (SP >= 64) ? 1 : 64 / SP;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -114,6 +116,25 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             if constexpr (NM) return ((size_t)node * (S / GS) + by) * (NFIELD * GS) + ls;
             else return (size_t)node * S + s;
         };
+        // Kahan terms: 6 fields in the same layout as the state
+        const size_t C6 = (size_t)6 * F;
+        const T *cin = KAHAN ? a.comp + (size_t)(t & 1) * C6 : nullptr;
+        T *cout = KAHAN ? a.comp + (size_t)((t + 1) & 1) * C6 : nullptr;
+        size_t co;
+        if constexpr (NM) co = ((size_t)row * (S / GS) + by) * (6 * GS) + ls;
+        else co = node_base(row);
+        // own-row data nobody else gathers: with CAP it is requested before the
+        // gathers so that its latency overlaps theirs
+        T pre_acc[6], pre_c[6];
+        if constexpr (CAP) {
+            const size_t o0 = node_base(row);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) pre_acc[q] = ld_once<NT>(in + (U1TT + q) * FS + o0);
+            if constexpr (KAHAN) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) pre_c[q] = ld_once<NT>(cin + q * FS + co);
+            }
+        }
 
         // ---- the 32 distinct products over one walk of the row ------------
         T s1u = 0, s1w = 0, s2u = 0, s2w = 0, s3u = 0, s3w = 0;
@@ -173,14 +194,14 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             mly3 = mac<EXACT>(mly3, am, ly3); mly4 = mac<EXACT>(mly4, am, ly4);
         };
         if constexpr (PACKED) {
-            // two nonzeros per trip: the gathers of both are in flight together;
-            // each sum still runs in column order
+            // UN nonzeros per trip: their gathers are in flight together; each
+            // sum still runs in column order
             int k = k0;
-            for (; k + 1 < k1; k += 2) {
-                nonzero(k);
-                nonzero(k + 1);
+            for (; k + UN - 1 < k1; k += UN) {
+#pragma unroll
+                for (int u = 0; u < UN; ++u) nonzero(k + u);
             }
-            if (k < k1) nonzero(k);
+            for (; k < k1; ++k) nonzero(k);
         } else {
             for (int k = k0; k < k1; ++k) nonzero(k);
         }
@@ -241,7 +262,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                 if (f < U1TT || f > W3TT) cur[f] = in[f * FS + o];
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) cur[U1TT + q] = ld_once<NT>(in + (U1TT + q) * FS + o);
+        for (int q = 0; q < 6; ++q) cur[U1TT + q] = CAP ? pre_acc[q] : ld_once<NT>(in + (U1TT + q) * FS + o);
 
         // ---- lumped-mass solve with Dirichlet clamp (:681-729) ------------
         T an[6], ln[8];
@@ -282,15 +303,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         for (int q = 0; q < 6; ++q) {
             const T d = cur[U1 + q], v = cur[U1T + q], acc = cur[U1TT + q];
             if constexpr (KAHAN) {
-                // Kahan terms: 6 fields in the same layout as the state
-                const size_t C6 = (size_t)6 * F;
-                const T *cin = a.comp + (size_t)(t & 1) * C6;
-                T *cout = a.comp + (size_t)((t + 1) & 1) * C6;
-                size_t co;
-                if constexpr (NM) co = ((size_t)row * (S / GS) + by) * (6 * GS) + ls;
-                else co = o;
                 const T inc = v * dt + (ca * acc + al * an[q]) * dt * dt;
-                const T y = inc - ld_once<NT>(cin + q * FS + co);
+                const T y = inc - (CAP ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
                 const T sum = d + y;
                 st_once<NT>(cout + q * FS + co, (sum - d) - y);
                 dn[q] = sum;
@@ -404,6 +418,8 @@ static hipError_t launch_step_t(int sp, bool cps, const void *args, dim3 g, int 
     case 1: return launch_step_v<T, PREC, 1>(sp, cps, a, g, block, slot, st);
     case 2: return launch_step_v<T, PREC, 2>(sp, cps, a, g, block, slot, st);
     case 3: return launch_step_v<T, PREC, 3>(sp, cps, a, g, block, slot, st);
+    case 4: return launch_step_v<T, PREC, 4>(sp, cps, a, g, block, slot, st);
+    case 5: return launch_step_v<T, PREC, 5>(sp, cps, a, g, block, slot, st);
     default: return hipErrorInvalidValue;
     }
 }
