@@ -69,7 +69,9 @@ def _case(tc, nelem, steps, nshots=1):
 @pytest.mark.parametrize("tc,nelem,steps,graph,guard,xcd,variant", [
     (host.Q4, 20, 150, True, 50, True, 1), (host.Q4, 20, 150, False, 7, False, 2),
     (host.Q9, 10, 120, True, 13, True, 1), (host.Q9, 10, 120, True, 50, True, 2),
-    (host.T6, 10, 120, True, 50, True, 1), (host.T3, 20, 120, False, 50, True, 1)])
+    (host.T6, 10, 120, True, 50, True, 1), (host.T3, 20, 120, False, 50, True, 1),
+    (host.Q4, 20, 150, True, 49, True, 4), (host.Q9, 10, 120, True, 50, False, 5),
+    (host.T6, 10, 120, True, 50, True, 6)])
 def test_fp64_loop_bit_identical(tc, nelem, steps, graph, guard, xcd, variant):
     case = _case(tc, nelem, steps)
     prob = case["problem"]
@@ -152,6 +154,30 @@ def test_row_order_changes_nothing(nshots, sp, prec):
         run = gpu.Run(op, md, case["src"], 60, np.arange(n, dtype=np.int32), case["dt"], case["f0"], case["t0"],
                       precision=prec, shots_per_lane_group=sp, guard_every=20, row_order=o)
         run.advance(60)
+        res.append(run.seismograms())
+        run.close()
+    assert np.max(np.abs(res[0][0])) > 0
+    for u, w in res[1:]:
+        assert bits_equal(u, res[0][0]) and bits_equal(w, res[0][1])
+    md.close()
+    op.close()
+
+
+@pytest.mark.parametrize("sp,prec", [(64, gpu.FP32_KAHAN), (32, gpu.FP32), (16, gpu.FP64)])
+def test_kernel_variants_agree_bit_for_bit(sp, prec):
+    """Every kernel variant (value packing, unroll depth, streaming hints,
+    diagonal capture) computes the same sums in the same order."""
+    case = _case(host.Q4, 12, 50, nshots=40)
+    prob = case["problem"]
+    n = prob["node_num"]
+    say(f"kernel variants 1-6: Q4 12x12 shots=40 sp={sp} precision={prec}")
+    op = gpu.Operator(n, prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    res = []
+    for v in range(1, 7):
+        run = gpu.Run(op, md, case["src"], 50, np.arange(n, dtype=np.int32), case["dt"], case["f0"], case["t0"],
+                      precision=prec, shots_per_lane_group=sp, guard_every=25, kernel_variant=v)
+        run.advance(50)
         res.append(run.seismograms())
         run.close()
     assert np.max(np.abs(res[0][0])) > 0
