@@ -142,8 +142,10 @@ typedef struct {
                                   arrays, 3 = 1 + streaming hints on data used once per step,
                                   4 = 3 + own-row fields kept from the diagonal gather and own-row
                                   accelerations / Kahan terms loaded first, 5 and 6 = 4 with 3 and 4
-                                  nonzeros' gathers in flight per trip.  Same results bit for bit;
-                                  a multi-run launch runs 3 to 6 as 1 */
+                                  nonzeros' gathers in flight per trip, 7 = LDS-staged gathers: each
+                                  block copies the fields of its rows' neighbour nodes into LDS once
+                                  (16, 32 or 64 shots per lane group only).  Same results bit for
+                                  bit; a multi-run launch runs 3 to 7 as 1 */
     const int *row_order;      /* [node_num] permutation: the order in which the step kernel's
                                   waves take rows (e.g. sfh_locality_order), NULL = node order.
                                   Changes only which rows run together, not any result */

@@ -24,8 +24,10 @@ def candidates(nshots):
     elif nshots <= 16:
         sps, blocks = [16, 64], [256]
     else:
-        sps, blocks = [32, 64], [256]
-    return list(itertools.product(sps, blocks, [1, 3, 4, 5, 6], [True, False], ["node", "hilbert"]))
+        sps, blocks = [16, 32, 64], [256]
+    # variant 7 (LDS-staged gathers) exists for lane groups of 16+ shots only
+    return [c for c in itertools.product(sps, blocks, [1, 3, 4, 5, 6, 7], [True, False], ["node", "hilbert"])
+            if c[2] != 7 or c[0] >= 16]
 
 
 def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None, coords=None):
@@ -43,9 +45,16 @@ def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None
         if oname not in orders:
             continue
         # two graph replays of `steps` launches: the first warms up, the second is timed
-        run = gpu.Run(op, md, src, 2 * steps, rec, dt, f0, t0, precision=precision, block=block,
-                      shots_per_lane_group=sp, kernel_variant=variant, xcd_remap=xcd, guard_every=steps,
-                      row_order=orders[oname])
+        try:
+            run = gpu.Run(op, md, src, 2 * steps, rec, dt, f0, t0, precision=precision, block=block,
+                          shots_per_lane_group=sp, kernel_variant=variant, xcd_remap=xcd, guard_every=steps,
+                          row_order=orders[oname])
+        except gpu.NativeError as e:
+            if e.code != -1:  # only an argument the shape cannot take (e.g. a halo beyond LDS) is skipped
+                raise
+            if log:
+                log(f"tune skip sp={sp} block={block} variant={variant} order={oname}: {e}")
+            continue
         run.advance(steps)
         run.advance(steps)
         ms = run.last_ms_per_step

@@ -208,6 +208,7 @@ extern "C" void abc_mpml(int node_num, int element_num, int element_order, int *
 struct sfg_operator {
     int n = 0, nnz = 0, device = 0;
     int *rp = nullptr, *ci = nullptr;
+    std::vector<int> hrp, hci;  // host copy of the union pattern (block halos)
     double *v64 = nullptr; // [7][nnz]
     float *v32 = nullptr;  // [7][nnz]
     double *p64 = nullptr; // [nnz][8]: M, S1..S6, 0 (one wide load per nonzero)
@@ -253,6 +254,8 @@ extern "C" int sfg_operator_create(sfg_operator **out, int n, const int *const c
     op->n = n;
     op->nnz = nnz;
     op->device = g_device;
+    op->hrp = up;
+    op->hci.assign(uj.begin(), uj.begin() + nnz);
     int rc = SFG_OK;
     if ((rc = dalloc(&op->rp, n + 1)) || (rc = dalloc(&op->ci, nnz)) ||
         (rc = dalloc(&op->v64, (size_t)7 * nnz)) || (rc = dalloc(&op->v32, (size_t)7 * nnz)) ||
@@ -387,6 +390,8 @@ struct sfg_run {
     Ctrl *ctrl = nullptr;
     int *active = nullptr, *src = nullptr, *rec_ptr = nullptr, *rec_idx = nullptr;
     int *order = nullptr;       // row_order on the device (or null)
+    int *hptr = nullptr, *hnode = nullptr, *kslot = nullptr, *rslot = nullptr; // LDS halos
+    size_t lds_bytes = 0;       // dynamic LDS of the LDS-staged kernel
     int *src_shot = nullptr;    // src = per-node entry pointers, then shot and weights per entry
     double *src_wt = nullptr;
     double *wavelet = nullptr, *seis_u = nullptr, *seis_w = nullptr, *energy = nullptr;
@@ -412,6 +417,10 @@ static void run_free(sfg_run *r)
     dfree(r->ctrl);
     dfree(r->active);
     dfree(r->order);
+    dfree(r->hptr);
+    dfree(r->hnode);
+    dfree(r->kslot);
+    dfree(r->rslot);
     dfree(r->src);
     dfree(r->src_shot);
     dfree(r->src_wt);
@@ -437,7 +446,7 @@ extern "C" void sfg_run_destroy(sfg_run *r)
 static int launch_one_step(sfg_run *r, int slot)
 {
     const void *args = (r->prec == 0) ? (const void *)&r->a64 : (const void *)&r->a32;
-    HIPCHK(launch_step(r->prec, r->SP, r->cps, args, r->nbx, r->nby, r->block, slot, r->stream));
+    HIPCHK(launch_step(r->prec, r->SP, r->cps, args, r->nbx, r->nby, r->block, slot, r->lds_bytes, r->stream));
     return SFG_OK;
 }
 
@@ -490,7 +499,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (!out || !op || !media || nmedia <= 0 || !d) return fail(SFG_E_ARG, "run: bad arguments");
     *out = nullptr;
     if (d->precision < 0 || d->precision > 2) return fail(SFG_E_ARG, "run: unknown precision %d", d->precision);
-    if (d->kernel_variant < 0 || d->kernel_variant > 6)
+    if (d->kernel_variant < 0 || d->kernel_variant > 7)
         return fail(SFG_E_ARG, "run: unknown kernel_variant %d", d->kernel_variant);
     if (d->nshots <= 0 || !d->src_node) return fail(SFG_E_ARG, "run: need >= 1 shot with a source node");
     if (nmedia != 1 && nmedia != d->nshots) return fail(SFG_E_ARG, "run: need one medium or one per shot");
@@ -630,6 +639,58 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     const size_t F = (size_t)n * r->S;
     const size_t w = r->prec == 0 ? 8 : 4;
     const size_t seis_n = (size_t)r->S * d->max_steps * std::max(d->rec_num, 1);
+
+    // LDS-staged gathers: the halo (distinct columns) of each block of rows
+    std::vector<int> hptr, hnode, kslot, rslot;
+    int lds_stride = 0;
+    if (d->kernel_variant == 7) {
+        if (!node_major(sp)) {
+            run_free(r);
+            return fail(SFG_E_ARG, "run: kernel_variant 7 (LDS-staged gathers) needs 16, 32 or 64 shots per lane group");
+        }
+        const int R = rows_per_block, gs = sp >= 64 ? 64 : sp;
+        hptr.assign(r->nbx + 1, 0);
+        kslot.assign(op->nnz, 0);
+        rslot.assign(n, 0);
+        std::vector<int> stamp(n, -1), slot_of(n, 0), blk;
+        size_t maxh = 0;
+        for (int b = 0; b < r->nbx; ++b) {
+            blk.clear();
+            auto add = [&](int x) { if (stamp[x] != b) { stamp[x] = b; blk.push_back(x); } };
+            const int l1 = std::min(n, (b + 1) * R);
+            for (int l = b * R; l < l1; ++l) {
+                const int i = d->row_order ? d->row_order[l] : l;
+                add(i);
+                for (int k = op->hrp[i]; k < op->hrp[i + 1]; ++k) add(op->hci[k]);
+            }
+            std::sort(blk.begin(), blk.end()); // ascending node ids: staging reads walk memory upwards
+            for (size_t h = 0; h < blk.size(); ++h) slot_of[blk[h]] = (int)h;
+            for (int l = b * R; l < l1; ++l) {
+                const int i = d->row_order ? d->row_order[l] : l;
+                rslot[i] = slot_of[i];
+                for (int k = op->hrp[i]; k < op->hrp[i + 1]; ++k) kslot[k] = slot_of[op->hci[k]];
+            }
+            hnode.insert(hnode.end(), blk.begin(), blk.end());
+            hptr[b + 1] = (int)hnode.size();
+            maxh = std::max(maxh, blk.size());
+        }
+        int lim = 0;
+        if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g_device) != hipSuccess) lim = 65536;
+        // pad each slot so the lane groups of one wave start on different LDS banks
+        lds_stride = 20 * gs + (gs < 64 ? gs : 0);
+        if (maxh * lds_stride * w > (size_t)lim) lds_stride = 20 * gs;
+        r->lds_bytes = maxh * lds_stride * w;
+        if (r->lds_bytes > (size_t)lim) {
+            run_free(r);
+            return fail(SFG_E_ARG, "run: a block's halo of %zu nodes needs %zu bytes of LDS (limit %d); use a "
+                        "smaller block or a locality row order", maxh, r->lds_bytes, lim);
+        }
+        hipError_t ea = allow_step_lds(r->prec, sp, r->cps, r->lds_bytes);
+        if (ea != hipSuccess) {
+            run_free(r);
+            return fail(SFG_E_HIP, "run: LDS size %zu: %s", r->lds_bytes, hipGetErrorString(ea));
+        }
+    }
     int rc = SFG_OK;
     hipError_t e = hipSuccess;
     if ((rc = dalloc((char **)&r->state, 2 * NFIELD * F * w)) ||
@@ -639,7 +700,9 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         (rc = dalloc(&r->rec_ptr, n + 1)) || (rc = dalloc(&r->rec_idx, ridx.size())) ||
         (rc = dalloc(&r->wavelet, d->max_steps)) || (rc = dalloc(&r->seis_u, seis_n)) ||
         (rc = dalloc(&r->seis_w, seis_n)) || (rc = dalloc(&r->energy, (size_t)r->nslots * r->S * 2)) ||
-        (d->row_order && (rc = dalloc(&r->order, n)))) {
+        (d->row_order && (rc = dalloc(&r->order, n))) ||
+        (!hptr.empty() && ((rc = dalloc(&r->hptr, hptr.size())) || (rc = dalloc(&r->hnode, hnode.size())) ||
+                           (rc = dalloc(&r->kslot, kslot.size())) || (rc = dalloc(&r->rslot, rslot.size()))))) {
         run_free(r);
         return rc;
     }
@@ -655,6 +718,10 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (e == hipSuccess) e = hipMemcpyAsync(r->active, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess && r->order)
         e = hipMemcpyAsync(r->order, d->row_order, sizeof(int) * n, hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess && r->hptr) e = hipMemcpyAsync(r->hptr, hptr.data(), sizeof(int) * hptr.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess && r->hnode) e = hipMemcpyAsync(r->hnode, hnode.data(), sizeof(int) * hnode.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess && r->kslot) e = hipMemcpyAsync(r->kslot, kslot.data(), sizeof(int) * kslot.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess && r->rslot) e = hipMemcpyAsync(r->rslot, rslot.data(), sizeof(int) * rslot.size(), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->src, sptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->src_shot, sshot.data(), sizeof(int) * sshot.size(), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->src_wt, swt.data(), sizeof(double) * swt.size(), hipMemcpyHostToDevice, r->stream);
@@ -708,10 +775,15 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.nbx = r->nbx;
         a.rp = op->rp;
         a.order = r->order;
+        a.hptr = r->hptr;
+        a.hnode = r->hnode;
+        a.kslot = r->kslot;
+        a.rslot = r->rslot;
+        a.lds_stride = lds_stride;
         a.ci = op->ci;
         for (int m = 0; m < 7; ++m) a.A[m] = vals + (size_t)m * op->nnz;
         // descriptor variant -> kernel variant (see step_tile)
-        static const int kv[7] = {1, 1, 0, 2, 3, 4, 5};
+        static const int kv[8] = {1, 1, 0, 2, 3, 4, 5, 6};
         a.variant = kv[d->kernel_variant];
         a.coef = r->cps ? (decltype(coef))r->coefp : coef;
         a.flag = r->cps ? r->flagp : md->flag;

@@ -78,6 +78,12 @@ struct StepArgs {
     int sp, cps, nbx;       // lane grouping, per-shot coefficients, row blocks (multi-run dispatch)
     const int *rp, *ci;     // union CSR pattern
     const int *order;       // row taken by logical row i (a permutation), or null = i
+    // LDS-staged gathers (node-major layouts): block b of rows stages the 20
+    // gathered fields of its halo nodes hnode[hptr[b] .. hptr[b+1]) into LDS,
+    // slot h at h * lds_stride; kslot[k] = slot of column ci[k] in the halo of
+    // the block holding its row, rslot[i] = slot of row i itself.
+    const int *hptr, *hnode, *kslot, *rslot;
+    int lds_stride;
     const T *A[7];          // M, S1..S6 on the union pattern (one array each)
     const T *Ap;            // the same values packed per nonzero: [nnz][8] = M, S1..S6, 0
     int variant;            // 0: separate value arrays; 1: packed values, gather loop unrolled x2
@@ -103,7 +109,8 @@ struct StepArgs {
 // Launchers (defined in sfg_kernels.hip).  precision: 0 exact FP64, 1 FP32, 2 FP32+Kahan.
 // slot: parity of the step being launched (see Ctrl).
 hipError_t launch_step(int precision, int sp, bool coef_per_shot, const void *args, int nblk_x,
-                       int nblk_y, int block, int slot, hipStream_t st);
+                       int nblk_y, int block, int slot, size_t lds_bytes, hipStream_t st);
+hipError_t allow_step_lds(int precision, int sp, bool coef_per_shot, size_t lds_bytes);
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
                              int block, int xcd_remap, int slot, hipStream_t st);
 hipError_t launch_guard(int precision, const void *state, int n, const StateLayout &L,

@@ -50,6 +50,15 @@ __device__ __forceinline__ void st_once(T *p, T v)
     else *p = v;
 }
 
+// 16-byte vector of T (staging copies into LDS)
+template <class T> struct vec16;
+template <> struct vec16<float> { using type = float4; };
+template <> struct vec16<double> { using type = double2; };
+
+// Position of state field f among the 20 gathered fields (U1..W3, U1t..W3t,
+// Lx1..Ly4; the accelerations are not gathered).
+__device__ __forceinline__ constexpr int gathered_slot(int f) { return f < U1TT ? f : f - 6; }
+
 // Workgroups are dispatched round-robin over the 8 XCDs; this map makes each
 // XCD walk one contiguous range of logical blocks (shot chunk, row block), so
 // neighbouring rows share that XCD's L2.  A bijection on [0, nb) for any nb.
@@ -77,16 +86,21 @@ __device__ __forceinline__ int begin_step(Ctrl *ctrl, int slot)
 // unrolled x2; 2 as 1 with streaming hints on data used once; 3 as 2, and the
 // row's own 20 gathered fields are kept from the diagonal nonzero instead of
 // being loaded again, and its accelerations and Kahan terms are loaded before
-// the gathers; 4 and 5 as 3 with 3 and 4 nonzeros' gathers in flight per trip.
+// the gathers; 4 and 5 as 3 with 3 and 4 nonzeros' gathers in flight per trip;
+// 6 (node-major layouts only) stages the 20 gathered fields of the block's halo
+// nodes into LDS once, and every row of the block gathers from there: each
+// node is read from L2 once per block instead of once per row that needs it.
 template <class T, int PREC, int SP, bool CPS, int V>
 __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, int by)
 {
     constexpr bool EXACT = (PREC == 0);
     constexpr bool KAHAN = (PREC == 2);
+    constexpr bool LDSG = V == 6;
     constexpr bool PACKED = V >= 1;
     constexpr bool NT = V >= 2;
-    constexpr bool CAP = V >= 3;
-    constexpr int UN = V >= 5 ? 4 : V == 4 ? 3 : 2;  // nonzeros per trip of the gather loop
+    constexpr bool PRE = V >= 3;                 // own-row accelerations / Kahan terms first
+    constexpr bool CAP = V >= 3 && !LDSG;        // own-row fields from the diagonal gather
+    constexpr int UN = V == 5 ? 4 : V == 4 ? 3 : 2;  // nonzeros per trip of the gather loop
     constexpr int RPW = (SP >= 64) ? 1 : 64 / SP;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -97,6 +111,28 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
     const int ls = SP >= 64 ? lane : lane % SP;  // this lane's shot within the group
     const int s = by * GS + ls;
     const int nact = (t < a.nsteps_max) ? a.active[t] : 0;
+    extern __shared__ __align__(16) unsigned char sfg_lds_raw[];
+    T *lds = reinterpret_cast<T *>(sfg_lds_raw);
+    if constexpr (LDSG) {
+        static_assert(node_major(SP), "LDS staging needs the node-major layout");
+        // a finished shot group skips the whole block (uniform, before the barrier)
+        if (by * GS >= nact) return;
+        using V16 = typename vec16<T>::type;
+        constexpr int VE = 16 / sizeof(T);
+        constexpr int GV = GS / VE;              // 16-byte vectors per field of one group
+        const T *inb = a.state + (size_t)(t & 1) * NFIELD * a.fstride;
+        const size_t NGr = (size_t)(a.S / GS);
+        const int h0 = a.hptr[bx], nh = a.hptr[bx + 1] - h0;
+        const int total = nh * 20 * GV;
+        for (int e = threadIdx.x; e < total; e += blockDim.x) {
+            const int v = e % GV, f20 = (e / GV) % 20, h = e / (GV * 20);
+            const int f = f20 < 12 ? f20 : f20 + 6;
+            const size_t g = ((size_t)a.hnode[h0 + h] * NGr + by) * (NFIELD * GS) + (size_t)f * GS + v * VE;
+            *reinterpret_cast<V16 *>(lds + h * a.lds_stride + f20 * GS + v * VE) =
+                *reinterpret_cast<const V16 *>(inb + g);
+        }
+        __syncthreads();
+    }
     if (row >= a.n) return;
     if (a.order) {
         // rows in locality order: the waves in flight hold a compact patch of the mesh
@@ -126,7 +162,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         // own-row data nobody else gathers: with CAP it is requested before the
         // gathers so that its latency overlaps theirs
         T pre_acc[6], pre_c[6];
-        if constexpr (CAP) {
+        if constexpr (PRE) {
             const size_t o0 = node_base(row);
 #pragma unroll
             for (int q = 0; q < 6; ++q) pre_acc[q] = ld_once<NT>(in + (U1TT + q) * FS + o0);
@@ -146,7 +182,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         T cur[NFIELD];
         bool got = false;
         auto nonzero = [&](int k) __attribute__((always_inline)) {
-            const int j = a.ci[k];
+            const int j = LDSG ? 0 : a.ci[k];
             T am, a1, a2, a3, a4, a5, a6;
             if constexpr (PACKED) {
                 const T *v = a.Ap + (size_t)k * 8;
@@ -156,13 +192,25 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                 a1 = a.A[1][k]; a2 = a.A[2][k]; a3 = a.A[3][k];
                 a4 = a.A[4][k]; a5 = a.A[5][k]; a6 = a.A[6][k];
             }
-            const T *x = in + node_base(j);
-            const T u1 = x[U1 * FS], u2 = x[U2 * FS], u3 = x[U3 * FS];
-            const T w1 = x[W1 * FS], w2 = x[W2 * FS], w3 = x[W3 * FS];
-            const T u1t = x[U1T * FS], u2t = x[U2T * FS], u3t = x[U3T * FS];
-            const T w1t = x[W1T * FS], w2t = x[W2T * FS], w3t = x[W3T * FS];
-            const T lx1 = x[LX1 * FS], lx2 = x[LX2 * FS], lx3 = x[LX3 * FS], lx4 = x[LX4 * FS];
-            const T ly1 = x[LY1 * FS], ly2 = x[LY2 * FS], ly3 = x[LY3 * FS], ly4 = x[LY4 * FS];
+            // field f of neighbour j: global state, or the block's LDS copy
+            const T *x;
+            size_t xs;
+            if constexpr (LDSG) {
+                x = lds + a.kslot[k] * a.lds_stride + ls;
+                xs = GS;
+            } else {
+                x = in + node_base(j);
+                xs = FS;
+            }
+            auto fld = [&](int f) __attribute__((always_inline)) {
+                return LDSG ? x[gathered_slot(f) * xs] : x[f * xs];
+            };
+            const T u1 = fld(U1), u2 = fld(U2), u3 = fld(U3);
+            const T w1 = fld(W1), w2 = fld(W2), w3 = fld(W3);
+            const T u1t = fld(U1T), u2t = fld(U2T), u3t = fld(U3T);
+            const T w1t = fld(W1T), w2t = fld(W2T), w3t = fld(W3T);
+            const T lx1 = fld(LX1), lx2 = fld(LX2), lx3 = fld(LX3), lx4 = fld(LX4);
+            const T ly1 = fld(LY1), ly2 = fld(LY2), ly3 = fld(LY3), ly4 = fld(LY4);
             if constexpr (CAP) {
                 if (j == row) {
                     cur[U1] = u1; cur[U2] = u2; cur[U3] = u3; cur[W1] = w1; cur[W2] = w2; cur[W3] = w3;
@@ -254,7 +302,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         const T rw7 = -dt * c33 * dyy * s6w - dt * dy * mly4 + mly4;
 
         // ---- own-row state ----------------------------------------------
-        if (!CAP || !got) {
+        if constexpr (LDSG) {
+            const T *x = lds + a.rslot[row] * a.lds_stride + ls;
+#pragma unroll
+            for (int f = 0; f < NFIELD; ++f)
+                if (f < U1TT || f > W3TT) cur[f] = x[gathered_slot(f) * GS];
+        } else if (!CAP || !got) {
             // (every row of an assembled operator holds its diagonal, so with
             // CAP this runs only for a pattern without one)
 #pragma unroll
@@ -262,7 +315,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                 if (f < U1TT || f > W3TT) cur[f] = in[f * FS + o];
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) cur[U1TT + q] = CAP ? pre_acc[q] : ld_once<NT>(in + (U1TT + q) * FS + o);
+        for (int q = 0; q < 6; ++q) cur[U1TT + q] = PRE ? pre_acc[q] : ld_once<NT>(in + (U1TT + q) * FS + o);
 
         // ---- lumped-mass solve with Dirichlet clamp (:681-729) ------------
         T an[6], ln[8];
@@ -304,7 +357,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             const T d = cur[U1 + q], v = cur[U1T + q], acc = cur[U1TT + q];
             if constexpr (KAHAN) {
                 const T inc = v * dt + (ca * acc + al * an[q]) * dt * dt;
-                const T y = inc - (CAP ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
+                const T y = inc - (PRE ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
                 const T sum = d + y;
                 st_once<NT>(cout + q * FS + co, (sum - d) - y);
                 dn[q] = sum;
@@ -394,9 +447,10 @@ hipError_t launch_multi_step(int precision, const void *args_dev, const int *til
 }
 
 template <class T, int PREC, int V>
-static hipError_t launch_step_v(int sp, bool cps, const StepArgs<T> &a, dim3 g, int block, int slot, hipStream_t st)
+static hipError_t launch_step_v(int sp, bool cps, const StepArgs<T> &a, dim3 g, int block, int slot, size_t lds,
+                                hipStream_t st)
 {
-#define SFG_L(SPV, C) hipLaunchKernelGGL((step_kernel<T, PREC, SPV, C, V>), g, dim3(block), 0, st, a, slot)
+#define SFG_L(SPV, C) hipLaunchKernelGGL((step_kernel<T, PREC, SPV, C, V>), g, dim3(block), lds, st, a, slot)
     switch (sp) {
     case 1: if (cps) SFG_L(1, true); else SFG_L(1, false); break;
     case 4: if (cps) SFG_L(4, true); else SFG_L(4, false); break;
@@ -409,29 +463,72 @@ static hipError_t launch_step_v(int sp, bool cps, const StepArgs<T> &a, dim3 g, 
     return hipGetLastError();
 }
 
+// LDS-staged variant: node-major lane groups only
 template <class T, int PREC>
-static hipError_t launch_step_t(int sp, bool cps, const void *args, dim3 g, int block, int slot, hipStream_t st)
+static hipError_t launch_step_lds(int sp, bool cps, const StepArgs<T> &a, dim3 g, int block, int slot, size_t lds,
+                                  hipStream_t st)
+{
+#define SFG_L(SPV, C) hipLaunchKernelGGL((step_kernel<T, PREC, SPV, C, 6>), g, dim3(block), lds, st, a, slot)
+    switch (sp) {
+    case 16: if (cps) SFG_L(16, true); else SFG_L(16, false); break;
+    case 32: if (cps) SFG_L(32, true); else SFG_L(32, false); break;
+    case 64: if (cps) SFG_L(64, true); else SFG_L(64, false); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef SFG_L
+    return hipGetLastError();
+}
+
+template <class T, int PREC>
+static hipError_t launch_step_t(int sp, bool cps, const void *args, dim3 g, int block, int slot, size_t lds,
+                                hipStream_t st)
 {
     const StepArgs<T> &a = *static_cast<const StepArgs<T> *>(args);
     switch (a.variant) {
-    case 0: return launch_step_v<T, PREC, 0>(sp, cps, a, g, block, slot, st);
-    case 1: return launch_step_v<T, PREC, 1>(sp, cps, a, g, block, slot, st);
-    case 2: return launch_step_v<T, PREC, 2>(sp, cps, a, g, block, slot, st);
-    case 3: return launch_step_v<T, PREC, 3>(sp, cps, a, g, block, slot, st);
-    case 4: return launch_step_v<T, PREC, 4>(sp, cps, a, g, block, slot, st);
-    case 5: return launch_step_v<T, PREC, 5>(sp, cps, a, g, block, slot, st);
+    case 0: return launch_step_v<T, PREC, 0>(sp, cps, a, g, block, slot, 0, st);
+    case 1: return launch_step_v<T, PREC, 1>(sp, cps, a, g, block, slot, 0, st);
+    case 2: return launch_step_v<T, PREC, 2>(sp, cps, a, g, block, slot, 0, st);
+    case 3: return launch_step_v<T, PREC, 3>(sp, cps, a, g, block, slot, 0, st);
+    case 4: return launch_step_v<T, PREC, 4>(sp, cps, a, g, block, slot, 0, st);
+    case 5: return launch_step_v<T, PREC, 5>(sp, cps, a, g, block, slot, 0, st);
+    case 6: return launch_step_lds<T, PREC>(sp, cps, a, g, block, slot, lds, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Allows the LDS-staged step kernel of this (precision, SP, per-shot
+// coefficients) to take lds_bytes of dynamic LDS.
+template <class T, int PREC>
+static hipError_t allow_lds_t(int sp, bool cps, int bytes)
+{
+    const void *f = nullptr;
+    switch (sp) {
+    case 16: f = cps ? (const void *)step_kernel<T, PREC, 16, true, 6> : (const void *)step_kernel<T, PREC, 16, false, 6>; break;
+    case 32: f = cps ? (const void *)step_kernel<T, PREC, 32, true, 6> : (const void *)step_kernel<T, PREC, 32, false, 6>; break;
+    case 64: f = cps ? (const void *)step_kernel<T, PREC, 64, true, 6> : (const void *)step_kernel<T, PREC, 64, false, 6>; break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+hipError_t allow_step_lds(int precision, int sp, bool cps, size_t lds_bytes)
+{
+    switch (precision) {
+    case 0: return allow_lds_t<double, 0>(sp, cps, (int)lds_bytes);
+    case 1: return allow_lds_t<float, 1>(sp, cps, (int)lds_bytes);
+    case 2: return allow_lds_t<float, 2>(sp, cps, (int)lds_bytes);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_step(int precision, int sp, bool cps, const void *args, int nbx, int nby,
-                       int block, int slot, hipStream_t st)
+                       int block, int slot, size_t lds_bytes, hipStream_t st)
 {
     const dim3 g(nbx, nby);
     switch (precision) {
-    case 0: return launch_step_t<double, 0>(sp, cps, args, g, block, slot, st);
-    case 1: return launch_step_t<float, 1>(sp, cps, args, g, block, slot, st);
-    case 2: return launch_step_t<float, 2>(sp, cps, args, g, block, slot, st);
+    case 0: return launch_step_t<double, 0>(sp, cps, args, g, block, slot, lds_bytes, st);
+    case 1: return launch_step_t<float, 1>(sp, cps, args, g, block, slot, lds_bytes, st);
+    case 2: return launch_step_t<float, 2>(sp, cps, args, g, block, slot, lds_bytes, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -9,7 +9,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._native import ShotDesc, check, lib
+from ._native import NativeError, ShotDesc, check, lib  # noqa: F401  (NativeError re-exported)
 
 FP64, FP32, FP32_KAHAN = 0, 1, 2
 PRECISION = {"fp64": FP64, "fp32": FP32, "fp32_kahan": FP32_KAHAN}

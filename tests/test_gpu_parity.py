@@ -166,7 +166,9 @@ def test_row_order_changes_nothing(nshots, sp, prec):
 @pytest.mark.parametrize("sp,prec", [(64, gpu.FP32_KAHAN), (32, gpu.FP32), (16, gpu.FP64)])
 def test_kernel_variants_agree_bit_for_bit(sp, prec):
     """Every kernel variant (value packing, unroll depth, streaming hints,
-    diagonal capture) computes the same sums in the same order."""
+    diagonal capture, LDS-staged gathers in node and Hilbert row order)
+    computes the same sums in the same order: seismograms at every node and
+    the final wavefield are bit-identical."""
     case = _case(host.Q4, 12, 50, nshots=40)
     prob = case["problem"]
     n = prob["node_num"]
@@ -174,15 +176,31 @@ def test_kernel_variants_agree_bit_for_bit(sp, prec):
     op = gpu.Operator(n, prob["csr"])
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
     res = []
-    for v in range(1, 7):
+    hil = host.locality_order(prob["x"], prob["y"])
+    for v, order in [(1, None), (2, None), (3, None), (4, None), (5, None), (6, None), (7, None), (7, hil)]:
         run = gpu.Run(op, md, case["src"], 50, np.arange(n, dtype=np.int32), case["dt"], case["f0"], case["t0"],
-                      precision=prec, shots_per_lane_group=sp, guard_every=25, kernel_variant=v)
+                      precision=prec, shots_per_lane_group=sp, guard_every=25, kernel_variant=v, row_order=order)
         run.advance(50)
         res.append(run.seismograms())
+        U, W = run.snapshot(7)
+        res[-1] = res[-1] + (U, W)
         run.close()
     assert np.max(np.abs(res[0][0])) > 0
-    for u, w in res[1:]:
-        assert bits_equal(u, res[0][0]) and bits_equal(w, res[0][1])
+    for r in res[1:]:
+        assert all(bits_equal(x, y) for x, y in zip(r, res[0]))
+    md.close()
+    op.close()
+
+
+def test_lds_variant_rejects_single_shot_groups():
+    case = _case(host.Q4, 8, 10)
+    prob = case["problem"]
+    say("LDS variant with SP=1 is refused")
+    op = gpu.Operator(prob["node_num"], prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    with pytest.raises(gpu.NativeError):
+        gpu.Run(op, md, case["src"], 10, case["rec"], case["dt"], case["f0"], case["t0"], kernel_variant=7,
+                shots_per_lane_group=1)
     md.close()
     op.close()
 
