@@ -78,8 +78,14 @@ def main():
             case = workloads.config2(steps_exec=args.steps or 4000)
             print(json.dumps(run_case("config2 Q9 400x400 6-layer VTI", case)), flush=True)
         elif c == "3":
-            case = workloads.config3(steps_exec=args.steps or 5000)
-            print(json.dumps(run_case("config3 T6 topography explosive", case, sources=case["sources"])), flush=True)
+            # T6 with M-PML carries a mode that grows ~1.3x per step (M_L^-1 M of the quadratic
+            # triangle exceeds 1; tests/test_oracle.py), so the reference loop itself grows without
+            # bound after ~2000 steps whatever dt is.  Compared over the first 1500 steps (0.3 s).
+            case = workloads.config3(steps_exec=args.steps or 1500)
+            out = run_case("config3 T6 topography explosive", case, sources=case["sources"])
+            out["note"] = ("first 1500 of the configured 5000 steps: the M-PML on T6 elements grows without "
+                           "bound after ~2000 steps in the FP64 reference loop as well")
+            print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -161,3 +161,27 @@ def test_symmetry_homogeneous_centre_source():
     assert np.max(np.abs(u - u[mirror])) < 1e-9 * s
     # w gets the truncated-pi cos(90 deg) force, so odd symmetry holds to that size
     assert np.max(np.abs(w + w[mirror])) < 1e-6 * max(s, np.max(np.abs(w)))
+
+
+@pytest.mark.parametrize("tc,unstable", [(host.Q4, False), (host.Q9, False), (host.T3, False), (host.T6, True)])
+def test_mpml_memory_recursion_amplifies_only_for_t6(tc, unstable):
+    """Inside the M-PML the memory fields follow L <- (1 - dt d) M_L^-1 M L + ...
+    (elastic_wave.c:694-722 with the products of :533-580): the consistent mass
+    divided by the lumped one.  Its spectrum reaches exactly 1 for Q4, Q9 and
+    T3 but 1.30 for T6 (negative corner couplings of the quadratic triangle),
+    so a T6 mesh with M-PML carries a mode growing ~1.3x per step whatever dt
+    is: configuration 3 (T6) grows without bound after ~2000 steps in FP64,
+    while the same mesh without M-PML stays bounded."""
+    en, x, y = host.mesh_structured(tc, 8, 8, 0.0, 80.0, 0.0, 80.0)
+    rho = np.full(x.size, 2000.0)
+    ci, cj, vals = host.assemble_coo(tc, en, x, y, rho)
+    p, j, v = host.coo2csr(x.size, ci, cj, vals[0])
+    M = np.zeros((x.size, x.size))
+    for i in range(x.size):
+        M[i, j[p[i]:p[i + 1]]] = v[p[i]:p[i + 1]]
+    lam = np.linalg.eigvals(M / host.lumped_mass(tc, en, x, y, rho)[:, None]).real
+    assert lam.min() > 0.0
+    if unstable:
+        assert lam.max() > 1.25
+    else:
+        assert abs(lam.max() - 1.0) < 1e-9
