@@ -124,12 +124,28 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         const size_t NGr = (size_t)(a.S / GS);
         const int h0 = a.hptr[bx], nh = a.hptr[bx + 1] - h0;
         const int total = nh * 20 * GV;
-        for (int e = threadIdx.x; e < total; e += blockDim.x) {
-            const int v = e % GV, f20 = (e / GV) % 20, h = e / (GV * 20);
-            const int f = f20 < 12 ? f20 : f20 + 6;
-            const size_t g = ((size_t)a.hnode[h0 + h] * NGr + by) * (NFIELD * GS) + (size_t)f * GS + v * VE;
-            *reinterpret_cast<V16 *>(lds + h * a.lds_stride + f20 * GS + v * VE) =
-                *reinterpret_cast<const V16 *>(inb + g);
+        // BATCH independent 16-byte loads per thread are in flight before their
+        // LDS stores, so the copy costs a few memory round trips, not one per element
+        constexpr int BATCH = 8;
+        const int nt = blockDim.x;
+        for (int e0 = threadIdx.x; e0 < total; e0 += BATCH * nt) {
+            V16 buf[BATCH];
+            int dst[BATCH];
+#pragma unroll
+            for (int b = 0; b < BATCH; ++b) {
+                const int e = e0 + b * nt;
+                dst[b] = -1;
+                if (e < total) {
+                    const int v = e % GV, f20 = (e / GV) % 20, h = e / (GV * 20);
+                    const int f = f20 < 12 ? f20 : f20 + 6;
+                    const size_t g = ((size_t)a.hnode[h0 + h] * NGr + by) * (NFIELD * GS) + (size_t)f * GS + v * VE;
+                    buf[b] = *reinterpret_cast<const V16 *>(inb + g);
+                    dst[b] = h * a.lds_stride + f20 * GS + v * VE;
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < BATCH; ++b)
+                if (dst[b] >= 0) *reinterpret_cast<V16 *>(lds + dst[b]) = buf[b];
         }
         __syncthreads();
     }
