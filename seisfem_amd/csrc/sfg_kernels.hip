@@ -131,21 +131,20 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         for (int e0 = threadIdx.x; e0 < total; e0 += BATCH * nt) {
             V16 buf[BATCH];
             int dst[BATCH];
+            // no branch per element (it would send the batch to scratch): an index
+            // past the end is clamped to the last element, whose copy is then
+            // stored twice with the same value
 #pragma unroll
             for (int b = 0; b < BATCH; ++b) {
-                const int e = e0 + b * nt;
-                dst[b] = -1;
-                if (e < total) {
-                    const int v = e % GV, f20 = (e / GV) % 20, h = e / (GV * 20);
-                    const int f = f20 < 12 ? f20 : f20 + 6;
-                    const size_t g = ((size_t)a.hnode[h0 + h] * NGr + by) * (NFIELD * GS) + (size_t)f * GS + v * VE;
-                    buf[b] = *reinterpret_cast<const V16 *>(inb + g);
-                    dst[b] = h * a.lds_stride + f20 * GS + v * VE;
-                }
+                const int e = min(e0 + b * nt, total - 1);
+                const int v = e % GV, f20 = (e / GV) % 20, h = e / (GV * 20);
+                const int f = f20 < 12 ? f20 : f20 + 6;
+                const size_t g = ((size_t)a.hnode[h0 + h] * NGr + by) * (NFIELD * GS) + (size_t)f * GS + v * VE;
+                buf[b] = *reinterpret_cast<const V16 *>(inb + g);
+                dst[b] = h * a.lds_stride + f20 * GS + v * VE;
             }
 #pragma unroll
-            for (int b = 0; b < BATCH; ++b)
-                if (dst[b] >= 0) *reinterpret_cast<V16 *>(lds + dst[b]) = buf[b];
+            for (int b = 0; b < BATCH; ++b) *reinterpret_cast<V16 *>(lds + dst[b]) = buf[b];
         }
         __syncthreads();
     }
