@@ -265,20 +265,34 @@ def main():
     Y, fem_s = gen.generate(models, batch=128, log=log)
     log(f"FEM data {Y.shape} in {time.time() - t0:.0f}s wall, {fem_s:.1f}s device")
     X = models["X"]
+    # the last tenth of the training models is held out for early stopping
+    nfit = args.train - args.train // 10
     net = MLP(NIN, args.hidden, 3, Y.shape[1], batch=args.batch, seed=args.seed)
-    net.set_data(X[:args.train], Y[:args.train])
-    steps_total = (args.train // args.batch) * args.epochs
+    net.set_data(X[:nfit], Y[:nfit])
+    steps_total = (nfit // args.batch) * args.epochs
     t1 = time.time()
-    loss, ms = net.train(args.epochs, lr=1e-3, total_steps=steps_total, seed=args.seed)
-    log(f"trained {args.epochs} epochs ({steps_total} steps) in {time.time() - t1:.1f}s, {ms:.3f} ms/step, "
-        f"loss {loss:.4g}")
+    chunk = max(1, args.epochs // 8)
+    best_val, epochs_done, ms_sum, nchunks = float("inf"), 0, 0.0, 0
+    while epochs_done < args.epochs:
+        loss, ms = net.train(chunk, lr=1e-3, total_steps=steps_total, seed=args.seed + epochs_done)
+        epochs_done += chunk
+        ms_sum, nchunks = ms_sum + ms, nchunks + 1
+        P, _ = net.predict(X[nfit:args.train])
+        val = float(np.median(rel_l2_rows(P, Y[nfit:args.train])))
+        log(f"epoch {epochs_done}: train loss {loss:.4g}, validation median rel-L2 {val:.3f}")
+        if val >= best_val:
+            break  # held-out error stopped falling
+        best_val = val
+    ms = ms_sum / nchunks
+    log(f"trained {epochs_done} epochs in {time.time() - t1:.1f}s, {ms:.3f} ms/step, loss {loss:.4g}")
     rep = evaluate(net, X[args.train:], Y[args.train:], fem_s / n, threshold=args.threshold)
     # what an untrained answer scores: the mean training trace for every test model
     mean_pred = np.broadcast_to(Y[:args.train].mean(axis=0), Y[args.train:].shape)
     rep["mean_trace_median_rel_l2"] = float(np.median(rel_l2_rows(mean_pred, Y[args.train:])))
     out = {"config": f"surrogate 13-{args.hidden}x3-{Y.shape[1]}, {args.train} FEM shots Q4 {args.nelem}x"
                      f"{args.nelem}, {args.steps} steps", "params": net.params, "train_ms_per_step": ms,
-           "final_train_mse": loss, "fem_device_s": fem_s, **rep}
+           "final_train_mse": loss, "epochs": epochs_done, "validation_median_rel_l2": best_val,
+           "fem_device_s": fem_s, **rep}
     print(json.dumps(out), flush=True)
     net.close()
     gen.close()
