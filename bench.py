@@ -183,10 +183,11 @@ def main():
         models = workloads.config4b(step_range=(args.steps // 2, args.steps))
         log(f"multi-model setup {time.time() - t_m:.1f}s: sizes {[c['size'] for c in models]}")
         keep, runs = [], []
-        # the multi-run kernel takes 1- or 64-shot lane groups; a lane's arithmetic does not
-        # depend on the grouping, so part 1 stays bit-identical to the single run
+        # the multi-run kernel takes lane groups of 1, 32 or 64 shots; a lane's arithmetic does
+        # not depend on the grouping, so part 1 stays bit-identical to the single run
+        sp1 = kw.get("shots_per_lane_group", 64)
         part1 = gpu.Run(op, md, case["src"], K, case["rec"], precision=prec, nsteps=nsteps,
-                        **dict(kw, shots_per_lane_group=64))
+                        **dict(kw, shots_per_lane_group=sp1 if sp1 in (32, 64) else 64))
         runs.append(part1)
         for c in models:
             p = c["problem"]

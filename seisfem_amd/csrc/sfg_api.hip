@@ -1085,7 +1085,8 @@ extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns
         if (r->prec != r0->prec || r->max_steps != r0->max_steps || r->block != r0->block ||
             r->guard_every != r0->guard_every)
             return fail(SFG_E_ARG, "multi: runs must share precision, max_steps, block and guard interval");
-        if (r->SP != 1 && r->SP != 64) return fail(SFG_E_ARG, "multi: runs need 1 or >= 64-shot lane groups");
+        if (r->SP != 1 && r->SP != 32 && r->SP != 64)
+            return fail(SFG_E_ARG, "multi: runs need lane groups of 1, 32 or 64 shots");
     }
     sfg_multi *m = new sfg_multi;
     m->prec = r0->prec;

@@ -415,7 +415,7 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a, int slot
 // step.  tile_start[r] is the first logical block of run r (runs in order,
 // tile_start[nruns] = total); a run's tiles are its (row block, shot chunk)
 // grid, row blocks fastest.  Runs with one row per lane (SP = 1) and runs with
-// one row per wave (SP = 64) can be mixed; every run shares the step counter
+// 32- or 64-shot lane groups can be mixed; every run shares the step counter
 // of args[0].ctrl, and a run past its last step has no active shot.
 template <class T, int PREC>
 __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__restrict__ args,
@@ -432,16 +432,24 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
     const StepArgs<T> &a = args[r];
     const int local = (int)bid - tile_start[r];
     const int bx = local % a.nbx, by = local / a.nbx;
-    // separate value arrays or packed values (the streaming-hint variants are
-    // run as packed here: one launch serves runs of every variant)
-    const bool pk = a.variant != 0;
+    // Kernel variants of one launch: separate value arrays (0), packed values
+    // (1; also runs 2, 3 and 6), and for 32- and 64-shot groups the unroll-3
+    // form with own-row data first (4; also runs 5).  Single-shot rows keep 0/1,
+    // whose register count leaves room for more waves.
+    const int v = a.variant;
+    const int w = v == 0 ? 0 : (v == 4 || v == 5) ? 4 : 1;
+#define SFG_T(SPV, C, VV) step_tile<T, PREC, SPV, C, VV>(a, t, bx, by)
     if (a.sp == 1) {
-        if (a.cps) { if (pk) step_tile<T, PREC, 1, true, 1>(a, t, bx, by); else step_tile<T, PREC, 1, true, 0>(a, t, bx, by); }
-        else { if (pk) step_tile<T, PREC, 1, false, 1>(a, t, bx, by); else step_tile<T, PREC, 1, false, 0>(a, t, bx, by); }
+        if (a.cps) { if (w) SFG_T(1, true, 1); else SFG_T(1, true, 0); }
+        else { if (w) SFG_T(1, false, 1); else SFG_T(1, false, 0); }
+    } else if (a.sp == 32) {
+        if (a.cps) { if (w == 4) SFG_T(32, true, 4); else if (w) SFG_T(32, true, 1); else SFG_T(32, true, 0); }
+        else { if (w == 4) SFG_T(32, false, 4); else if (w) SFG_T(32, false, 1); else SFG_T(32, false, 0); }
     } else {
-        if (a.cps) { if (pk) step_tile<T, PREC, 64, true, 1>(a, t, bx, by); else step_tile<T, PREC, 64, true, 0>(a, t, bx, by); }
-        else { if (pk) step_tile<T, PREC, 64, false, 1>(a, t, bx, by); else step_tile<T, PREC, 64, false, 0>(a, t, bx, by); }
+        if (a.cps) { if (w == 4) SFG_T(64, true, 4); else if (w) SFG_T(64, true, 1); else SFG_T(64, true, 0); }
+        else { if (w == 4) SFG_T(64, false, 4); else if (w) SFG_T(64, false, 1); else SFG_T(64, false, 0); }
     }
+#undef SFG_T
 }
 
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,

@@ -250,12 +250,13 @@ def test_per_shot_media_match_their_own_oracle_runs():
 
 
 def test_multi_run_one_launch_per_step_matches_separate_runs():
-    """Three runs on different meshes (single-shot and 70-shot lane groups,
-    uneven step counts) stepped by one launch per step: FP64 seismograms
-    bit-identical to the oracle run of each shot."""
-    specs = [(10, 1, 90), (13, 1, 55), (12, 70, 90)]
+    """Four runs on different meshes (single-shot, 70 shots in 64-shot groups,
+    45 shots in 32-shot groups with the unroll-3 variant; uneven step counts)
+    stepped by one launch per step: FP64 seismograms bit-identical to the
+    oracle run of each shot."""
+    specs = [(10, 1, 90, 0, 0), (13, 1, 55, 0, 0), (12, 70, 90, 64, 1), (11, 45, 90, 32, 5)]
     cases, runs, keep = [], [], []
-    for nelem, nshots, steps in specs:
+    for nelem, nshots, steps, sp, variant in specs:
         case = _case(host.Q4, nelem, 90, nshots=nshots)
         prob = case["problem"]
         op = gpu.Operator(prob["node_num"], prob["csr"])
@@ -266,11 +267,11 @@ def test_multi_run_one_launch_per_step_matches_separate_runs():
             nsteps[0] = 90
         # one launch has one block size: the runs of a group share it
         run = gpu.Run(op, md, case["src"], 90, case["rec"], case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
-                      nsteps=nsteps, guard_every=25, block=256)
+                      nsteps=nsteps, guard_every=25, block=256, shots_per_lane_group=sp, kernel_variant=variant)
         cases.append((case, nsteps))
         runs.append(run)
         keep += [op, md]
-    say("multi-run: Q4 10x10 (1 shot), 13x13 (1 shot, 55 steps), 12x12 (70 shots, uneven)")
+    say("multi-run: Q4 10x10 (1 shot), 13x13 (1 shot, 55 steps), 12x12 (70 shots), 11x11 (45 shots, sp 32)")
     multi = gpu.Multi(runs)
     multi.advance(90)
     for (case, nsteps), run in zip(cases, runs):
