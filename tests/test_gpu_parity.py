@@ -360,3 +360,43 @@ def test_energy_guard_reports_instead_of_exiting():
     run.close()
     md.close()
     op.close()
+
+
+def test_edge_cases_no_receivers_zero_steps_no_source():
+    """A run without receivers; shots that execute 0 steps or have no source
+    (node -1) stay exactly zero while their neighbours in the same run match
+    the oracle; a one-step run."""
+    case = _case(host.Q4, 10, 40, nshots=4)
+    prob = case["problem"]
+    n = prob["node_num"]
+    op = gpu.Operator(n, prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    say("edge cases: no receivers; shots with 0 steps / no source; one step")
+    r0 = gpu.Run(op, md, case["src"], 40, np.zeros(0, np.int32), case["dt"], case["f0"], case["t0"],
+                 precision=gpu.FP64)
+    r0.advance(40)
+    U, W = r0.snapshot(1)
+    assert np.max(np.abs(U)) > 0
+    r0.close()
+    src = np.array([case["src"][0], -1, case["src"][2], case["src"][3]], np.int32)
+    nsteps = np.array([40, 40, 0, 17], np.int32)
+    allnodes = np.arange(n, dtype=np.int32)
+    r1 = gpu.Run(op, md, src, 40, allnodes, case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
+                 nsteps=nsteps, shots_per_lane_group=4, guard_every=10)
+    r1.advance(40)
+    u, w = r1.seismograms()
+    assert not np.any(u[1]) and not np.any(w[1])           # no source
+    assert not np.any(u[2]) and not np.any(w[2])           # no step executed
+    for s in (0, 3):
+        ref = oracle.oracle_masslump(prob, src[s], int(nsteps[s]) + 2, allnodes, case["dt"], case["f0"], case["t0"])
+        k = int(nsteps[s])
+        assert bits_equal(u[s, :k], ref["seis_u"]) and bits_equal(w[s, :k], ref["seis_w"])
+    r1.close()
+    r2 = gpu.Run(op, md, case["src"][:1], 1, allnodes, case["dt"], case["f0"], case["t0"], precision=gpu.FP64)
+    assert r2.advance(5) == 1                               # clamps to max_steps
+    u, w = r2.seismograms()
+    ref = oracle.oracle_masslump(prob, case["src"][0], 3, allnodes, case["dt"], case["f0"], case["t0"])
+    assert bits_equal(u[0, :1], ref["seis_u"]) and bits_equal(w[0, :1], ref["seis_w"])
+    r2.close()
+    md.close()
+    op.close()
