@@ -401,12 +401,22 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
 
 }
 
+// The grid holds every (row block, shot group) tile; a launch hands out only
+// the tiles of the shot groups still active at its step (shots are sorted by
+// decreasing step count, so they are the first groups) and the surplus blocks
+// exit at once.  With the XCD remap this keeps the remaining work spread over
+// all 8 XCDs when shots finish, instead of leaving it on the XCDs that own the
+// first groups.
 template <class T, int PREC, int SP, bool CPS, int V>
 __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a, int slot)
 {
     const int t = begin_step(a.ctrl, slot);
-    const unsigned int nb = gridDim.x * gridDim.y;
+    constexpr unsigned int GS = SP >= 64 ? 64 : SP;
+    const unsigned int nact = (t < a.nsteps_max) ? (unsigned int)a.active[t] : 0u;
+    const unsigned int groups = min(gridDim.y, (nact + GS - 1) / GS);
+    const unsigned int nb = gridDim.x * groups;
     unsigned int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    if (bid >= nb) return;
     if (a.xcd_remap) bid = xcd_contiguous(bid, nb);
     step_tile<T, PREC, SP, CPS, V>(a, t, (int)(bid % gridDim.x), (int)(bid / gridDim.x));
 }
