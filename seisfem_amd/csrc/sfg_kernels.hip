@@ -443,10 +443,11 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
     unsigned int bid = blockIdx.x;
     if (bid >= nb) return;
     if (xcd_remap) bid = xcd_contiguous(bid, nb);
-    // run of this tile: a wave-uniform scan of the prefix row (scalar loads;
-    // the first run, which holds most tiles, is found at the first compare)
-    int r = 0;
-    while (r + 1 < nruns && bid >= (unsigned int)tp[r + 1]) ++r;
+    // run of this tile: the number of runs that end at or before it (one load
+    // per lane and a ballot; nruns <= 64)
+    const int lane = threadIdx.x & 63;
+    const unsigned int end = lane < nruns ? (unsigned int)tp[lane + 1] : 0xffffffffu;
+    const int r = __popcll(__ballot(bid >= end));
     const StepArgs<T> &a = args[r];
     const int local = (int)(bid - (unsigned int)tp[r]);
     const int bx = local % a.nbx, by = local / a.nbx;
