@@ -1035,7 +1035,7 @@ struct sfg_multi {
     int prec = 0, block = 256, ntiles = 0, max_steps = 0, guard_every = 50, executed = 0;
     Ctrl *ctrl = nullptr;       // shared step counter
     void *args = nullptr;       // device StepArgs<T>[nruns]
-    int *tile_prefix = nullptr; // device [max_steps][nruns + 1]: active tiles of runs 0..r-1 at step t
+    int *tile_start = nullptr;  // device [nruns + 1]
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // by parity of the first step
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1052,13 +1052,13 @@ static void multi_free(sfg_multi *m)
     if (m->stream) (void)hipStreamDestroy(m->stream);
     dfree(m->ctrl);
     dfree(m->args);
-    dfree(m->tile_prefix);
+    dfree(m->tile_start);
     delete m;
 }
 
 static int multi_launch_step(sfg_multi *m, int slot)
 {
-    HIPCHK(launch_multi_step(m->prec, m->args, m->tile_prefix, (int)m->runs.size(), m->ntiles, m->block, 1, slot,
+    HIPCHK(launch_multi_step(m->prec, m->args, m->tile_start, (int)m->runs.size(), m->ntiles, m->block, 1, slot,
                              m->stream));
     return SFG_OK;
 }
@@ -1076,7 +1076,6 @@ extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns
 {
     g_err.clear();
     if (!out || !runs || nruns <= 0) return fail(SFG_E_ARG, "multi: bad arguments");
-    if (nruns > 64) return fail(SFG_E_ARG, "multi: at most 64 runs per group");
     *out = nullptr;
     const sfg_run *r0 = runs[0];
     for (int k = 0; k < nruns; ++k) {
@@ -1094,29 +1093,16 @@ extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns
     m->block = r0->block;
     m->max_steps = r0->max_steps;
     m->guard_every = r0->guard_every;
-    m->ntiles = 0;
+    std::vector<int> ts(nruns + 1, 0);
     for (int k = 0; k < nruns; ++k) {
         m->runs.push_back(runs[k]);
-        m->ntiles += runs[k]->nbx * runs[k]->nby;
+        ts[k + 1] = ts[k] + runs[k]->nbx * runs[k]->nby;
     }
-    // per step: prefix sums over the runs of the tiles whose shot group still
-    // has an active shot (shots of a run are sorted by decreasing step count)
-    std::vector<int> tp((size_t)m->max_steps * (nruns + 1), 0);
-    for (int k = 0; k < nruns; ++k) {
-        const sfg_run *r = runs[k];
-        const int gs = r->SP >= 64 ? 64 : r->SP;
-        int act = r->nshots;
-        for (int t = 0; t < m->max_steps; ++t) {
-            while (act > 0 && r->steps_int[act - 1] <= t) --act;
-            const int groups = std::min(r->nby, (act + gs - 1) / gs);
-            int *row = tp.data() + (size_t)t * (nruns + 1);
-            row[k + 1] = row[k] + r->nbx * groups;
-        }
-    }
+    m->ntiles = ts[nruns];
     const size_t asz = m->prec == 0 ? sizeof(StepArgs<double>) : sizeof(StepArgs<float>);
     int rc = SFG_OK;
     if ((rc = dalloc(&m->ctrl, 1)) || (rc = dalloc((char **)&m->args, asz * nruns)) ||
-        (rc = dalloc(&m->tile_prefix, tp.size()))) {
+        (rc = dalloc(&m->tile_start, nruns + 1))) {
         multi_free(m);
         return rc;
     }
@@ -1138,7 +1124,7 @@ extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns
     if (e == hipSuccess) e = hipEventCreate(&m->ev1);
     if (e == hipSuccess) e = launch_fill_zero(m->ctrl, sizeof(Ctrl), m->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(m->args, tab.data(), tab.size(), hipMemcpyHostToDevice, m->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(m->tile_prefix, tp.data(), sizeof(int) * tp.size(), hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(m->tile_start, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice, m->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
     if (e != hipSuccess) {
         multi_free(m);

@@ -111,7 +111,7 @@ struct StepArgs {
 hipError_t launch_step(int precision, int sp, bool coef_per_shot, const void *args, int nblk_x,
                        int nblk_y, int block, int slot, size_t lds_bytes, hipStream_t st);
 hipError_t allow_step_lds(int precision, int sp, bool coef_per_shot, size_t lds_bytes);
-hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_prefix_dev, int nruns, int ntiles,
+hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
                              int block, int xcd_remap, int slot, hipStream_t st);
 hipError_t launch_guard(int precision, const void *state, int n, const StateLayout &L,
                         const Ctrl *ctrl, double *energy, int nslots, int every, hipStream_t st);

@@ -422,35 +422,42 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a, int slot
 }
 
 // Several runs (own mesh, medium, shots and step counts) in one launch per
-// step.  A run's tiles are its (row block, shot chunk) grid, row blocks
-// fastest; runs follow each other in order.  Runs with one row per lane (SP = 1) and runs with
+// step.  tile_start[r] is the first logical block of run r (runs in order,
+// tile_start[nruns] = total); a run's tiles are its (row block, shot chunk)
+// grid, row blocks fastest.  Runs with one row per lane (SP = 1) and runs with
 // 32- or 64-shot lane groups can be mixed; every run shares the step counter
 // of args[0].ctrl, and a run past its last step has no active shot.
 template <class T, int PREC>
 __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__restrict__ args,
-                                                         const int *__restrict__ tile_prefix, int nruns,
+                                                         const int *__restrict__ tile_start, int nruns,
                                                          int xcd_remap, int slot)
 {
     Ctrl *ctrl = args[0].ctrl;
     const int t = begin_step(ctrl, slot);
-    // tile_prefix[t][r] = tiles of the shot groups still active at step t in
-    // runs 0..r-1 (built at group creation from the fixed step counts); a
-    // launch hands out only those tiles and remaps them over the XCDs, so
-    // finished runs and shot groups leave no XCD idle.
-    if (t >= args[0].nsteps_max) return;
-    const int *tp = tile_prefix + (size_t)t * (nruns + 1);
-    const unsigned int nb = (unsigned int)tp[nruns];
+    // The grid holds every tile (tile_start); a launch hands out only the tiles
+    // of the shot groups still active at its step, run by run, and remaps
+    // those over the XCDs, so finished runs and shot groups leave no XCD idle.
+    auto active_tiles = [&](const StepArgs<T> &q) __attribute__((always_inline)) -> unsigned int {
+        const unsigned int gs = q.sp >= 64 ? 64u : (unsigned int)q.sp;
+        const unsigned int na = t < q.nsteps_max ? (unsigned int)q.active[t] : 0u;
+        return (unsigned int)q.nbx * min((unsigned int)q.S / gs, (na + gs - 1) / gs);
+    };
+    unsigned int nb = 0;
+    for (int q = 0; q < nruns; ++q) nb += active_tiles(args[q]);
     unsigned int bid = blockIdx.x;
     if (bid >= nb) return;
     if (xcd_remap) bid = xcd_contiguous(bid, nb);
-    // run of this tile: the number of runs that end at or before it (one load
-    // per lane and a ballot; nruns <= 64)
-    const int lane = threadIdx.x & 63;
-    const unsigned int end = lane < nruns ? (unsigned int)tp[lane + 1] : 0xffffffffu;
-    const int r = __popcll(__ballot(bid >= end));
+    int r = 0;
+    unsigned int base = 0;
+    for (; r < nruns - 1; ++r) {
+        const unsigned int n_r = active_tiles(args[r]);
+        if (bid < base + n_r) break;
+        base += n_r;
+    }
     const StepArgs<T> &a = args[r];
-    const int local = (int)(bid - (unsigned int)tp[r]);
+    const int local = (int)(bid - base);
     const int bx = local % a.nbx, by = local / a.nbx;
+    (void)tile_start;
     // Kernel variants of one launch: separate value arrays (0), packed values
     // (1; also runs 2, 3 and 6), and for 32- and 64-shot groups the unroll-3
     // form with own-row data first (4; also runs 5).  Single-shot rows keep 0/1,
@@ -471,18 +478,18 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
 #undef SFG_T
 }
 
-hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_prefix_dev, int nruns, int ntiles,
+hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
                              int block, int xcd_remap, int slot, hipStream_t st)
 {
     if (precision == 0)
         hipLaunchKernelGGL((multi_step_kernel<double, 0>), dim3(ntiles), dim3(block), 0, st,
-                           (const StepArgs<double> *)args_dev, tile_prefix_dev, nruns, xcd_remap, slot);
+                           (const StepArgs<double> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
     else if (precision == 1)
         hipLaunchKernelGGL((multi_step_kernel<float, 1>), dim3(ntiles), dim3(block), 0, st,
-                           (const StepArgs<float> *)args_dev, tile_prefix_dev, nruns, xcd_remap, slot);
+                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
     else if (precision == 2)
         hipLaunchKernelGGL((multi_step_kernel<float, 2>), dim3(ntiles), dim3(block), 0, st,
-                           (const StepArgs<float> *)args_dev, tile_prefix_dev, nruns, xcd_remap, slot);
+                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
