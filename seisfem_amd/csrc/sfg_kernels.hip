@@ -434,30 +434,14 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
 {
     Ctrl *ctrl = args[0].ctrl;
     const int t = begin_step(ctrl, slot);
-    // The grid holds every tile (tile_start); a launch hands out only the tiles
-    // of the shot groups still active at its step, run by run, and remaps
-    // those over the XCDs, so finished runs and shot groups leave no XCD idle.
-    auto active_tiles = [&](const StepArgs<T> &q) __attribute__((always_inline)) -> unsigned int {
-        const unsigned int gs = q.sp >= 64 ? 64u : (unsigned int)q.sp;
-        const unsigned int na = t < q.nsteps_max ? (unsigned int)q.active[t] : 0u;
-        return (unsigned int)q.nbx * min((unsigned int)q.S / gs, (na + gs - 1) / gs);
-    };
-    unsigned int nb = 0;
-    for (int q = 0; q < nruns; ++q) nb += active_tiles(args[q]);
+    const unsigned int nb = gridDim.x;
     unsigned int bid = blockIdx.x;
-    if (bid >= nb) return;
     if (xcd_remap) bid = xcd_contiguous(bid, nb);
     int r = 0;
-    unsigned int base = 0;
-    for (; r < nruns - 1; ++r) {
-        const unsigned int n_r = active_tiles(args[r]);
-        if (bid < base + n_r) break;
-        base += n_r;
-    }
+    while (r + 1 < nruns && (int)bid >= tile_start[r + 1]) ++r;
     const StepArgs<T> &a = args[r];
-    const int local = (int)(bid - base);
+    const int local = (int)bid - tile_start[r];
     const int bx = local % a.nbx, by = local / a.nbx;
-    (void)tile_start;
     // Kernel variants of one launch: separate value arrays (0), packed values
     // (1; also runs 2, 3 and 6), and for 32- and 64-shot groups the unroll-3
     // form with own-row data first (4; also runs 5).  Single-shot rows keep 0/1,
