@@ -68,7 +68,7 @@ def main():
     ap.add_argument("--launch64", default="", help="pin the FP64 single-shot launch shape (same form)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+        print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); "
               "shots would shard across GPUs with no communication, but no multi-GPU figure is produced.",
               file=sys.stderr)
         sys.exit(2)

@@ -52,3 +52,11 @@ def test_stream_copy_rejects_bad_sizes_before_touching_the_device():
         with pytest.raises(NativeError) as e:
             gpu.stream_copy(nbytes, reps)
         assert e.value.code == -1
+
+
+def test_bench_refuses_more_than_one_gpu_without_touching_it():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
