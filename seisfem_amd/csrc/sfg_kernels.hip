@@ -50,6 +50,16 @@ __device__ __forceinline__ void st_once(T *p, T v)
     else *p = v;
 }
 
+// Read-only arrays of a step (pattern, values, coefficients, schedules) are
+// read through the constant address space: they do not change during a
+// launch, which lets wave-uniform reads (one row per wave) go through the
+// scalar unit instead of the vector-memory path the gathers saturate.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *ro(const T *p)
+{
+    return (const __attribute__((address_space(4))) T *)p;
+}
+
 // 16-byte vector of T (staging copies into LDS)
 template <class T> struct vec16;
 template <> struct vec16<float> { using type = float4; };
@@ -110,7 +120,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
     constexpr int GS = SP >= 64 ? 64 : SP;      // shots of one lane group
     const int ls = SP >= 64 ? lane : lane % SP;  // this lane's shot within the group
     const int s = by * GS + ls;
-    const int nact = (t < a.nsteps_max) ? a.active[t] : 0;
+    const int nact = (t < a.nsteps_max) ? ro(a.active)[t] : 0;
     extern __shared__ __align__(16) unsigned char sfg_lds_raw[];
     T *lds = reinterpret_cast<T *>(sfg_lds_raw);
     if constexpr (LDSG) {
@@ -151,7 +161,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
     if (row >= a.n) return;
     if (a.order) {
         // rows in locality order: the waves in flight hold a compact patch of the mesh
-        row = a.order[row];
+        row = ro(a.order)[row];
         if constexpr (SP >= 64) row = __builtin_amdgcn_readfirstlane(row);
     }
 
@@ -192,26 +202,26 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         T s4u = 0, s4w = 0, s5u = 0, s5w = 0, s6u = 0, s6w = 0;
         T mu1t = 0, mu1 = 0, mlx1 = 0, mlx2 = 0, mu2t = 0, mu2 = 0, mu3t = 0, mu3 = 0, mlx3 = 0, mlx4 = 0;
         T mw1t = 0, mw1 = 0, mly1 = 0, mly2 = 0, mw2t = 0, mw2 = 0, mw3t = 0, mw3 = 0, mly3 = 0, mly4 = 0;
-        const int k0 = a.rp[row], k1 = a.rp[row + 1];
+        const int k0 = ro(a.rp)[row], k1 = ro(a.rp)[row + 1];
         const size_t o = node_base(row);
         T cur[NFIELD];
         bool got = false;
         auto nonzero = [&](int k) __attribute__((always_inline)) {
-            const int j = LDSG ? 0 : a.ci[k];
+            const int j = LDSG ? 0 : ro(a.ci)[k];
             T am, a1, a2, a3, a4, a5, a6;
             if constexpr (PACKED) {
-                const T *v = a.Ap + (size_t)k * 8;
+                const auto v = ro(a.Ap) + (size_t)k * 8;
                 am = v[0]; a1 = v[1]; a2 = v[2]; a3 = v[3]; a4 = v[4]; a5 = v[5]; a6 = v[6];
             } else {
-                am = a.A[0][k];
-                a1 = a.A[1][k]; a2 = a.A[2][k]; a3 = a.A[3][k];
-                a4 = a.A[4][k]; a5 = a.A[5][k]; a6 = a.A[6][k];
+                am = ro(a.A[0])[k];
+                a1 = ro(a.A[1])[k]; a2 = ro(a.A[2])[k]; a3 = ro(a.A[3])[k];
+                a4 = ro(a.A[4])[k]; a5 = ro(a.A[5])[k]; a6 = ro(a.A[6])[k];
             }
             // field f of neighbour j: global state, or the block's LDS copy
             const T *x;
             size_t xs;
             if constexpr (LDSG) {
-                x = lds + a.kslot[k] * a.lds_stride + ls;
+                x = lds + ro(a.kslot)[k] * a.lds_stride + ls;
                 xs = GS;
             } else {
                 x = in + node_base(j);
@@ -272,16 +282,17 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         // ---- per-node coefficients --------------------------------------
         const size_t cs = CPS ? (size_t)a.n * S : (size_t)a.n;
         const size_t ci = CPS ? (size_t)row * S + s : (size_t)row;
-        const T c11 = a.coef[C11 * cs + ci], c13 = a.coef[C13 * cs + ci];
-        const T c33 = a.coef[C33 * cs + ci], c44 = a.coef[C44 * cs + ci];
-        const T dx = a.coef[DX * cs + ci], dy = a.coef[DY * cs + ci];
-        const T dxx = a.coef[DXX * cs + ci], dyy = a.coef[DYY * cs + ci];
-        const T dxxp = a.coef[DXXP * cs + ci], dyyp = a.coef[DYYP * cs + ci];
-        const T mass = a.coef[MASS * cs + ci];
+        const auto cf = ro(a.coef);
+        const T c11 = cf[C11 * cs + ci], c13 = cf[C13 * cs + ci];
+        const T c33 = cf[C33 * cs + ci], c44 = cf[C44 * cs + ci];
+        const T dx = cf[DX * cs + ci], dy = cf[DY * cs + ci];
+        const T dxx = cf[DXX * cs + ci], dyy = cf[DYY * cs + ci];
+        const T dxxp = cf[DXXP * cs + ci], dyyp = cf[DYYP * cs + ci];
+        const T mass = cf[MASS * cs + ci];
         const T dt = a.dt;
         if constexpr (CPS) {
             // per-shot density: scale this row of the operator's M
-            const T mr = a.coef[MROW * cs + ci];
+            const T mr = cf[MROW * cs + ci];
             mu1t *= mr; mu1 *= mr; mlx1 *= mr; mlx2 *= mr; mu2t *= mr; mu2 *= mr;
             mu3t *= mr; mu3 *= mr; mlx3 *= mr; mlx4 *= mr;
             mw1t *= mr; mw1 *= mr; mly1 *= mr; mly2 *= mr; mw2t *= mr; mw2 *= mr;
@@ -290,12 +301,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
 
         // source term: the reference adds (i == source_node) * point_source * sin|cos(angle)
         // to every row, so a row without a source still adds (0 * ps) * factor
-        const double ps = a.wavelet[t];
+        const double ps = ro(a.wavelet)[t];
         double su = 0.0 * ps * a.src_u, sw = 0.0 * ps * a.src_w;
-        for (int q = a.src_ptr[row]; q < a.src_ptr[row + 1]; ++q)
-            if (a.src_shot[q] == s) {
-                su = 1.0 * ps * a.src_wt[2 * q];
-                sw = 1.0 * ps * a.src_wt[2 * q + 1];
+        for (int q = ro(a.src_ptr)[row]; q < ro(a.src_ptr)[row + 1]; ++q)
+            if (ro(a.src_shot)[q] == s) {
+                su = 1.0 * ps * ro(a.src_wt)[2 * q];
+                sw = 1.0 * ps * ro(a.src_wt)[2 * q + 1];
             }
         const T fu = (T)su;
         const T fw = (T)sw;
@@ -318,7 +329,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
 
         // ---- own-row state ----------------------------------------------
         if constexpr (LDSG) {
-            const T *x = lds + a.rslot[row] * a.lds_stride + ls;
+            const T *x = lds + ro(a.rslot)[row] * a.lds_stride + ls;
 #pragma unroll
             for (int f = 0; f < NFIELD; ++f)
                 if (f < U1TT || f > W3TT) cur[f] = x[gathered_slot(f) * GS];
@@ -336,7 +347,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         T an[6], ln[8];
         if constexpr (EXACT) {
             if (mass != T(0)) {
-                if (a.flag[ci] == 1) {
+                if (ro(a.flag)[ci] == 1) {
 #pragma unroll
                     for (int q = 0; q < 6; ++q) an[q] = T(0);
 #pragma unroll
@@ -387,12 +398,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         for (int q = 0; q < 8; ++q) st_once<NT>(out + (LX1 + q) * FS + o, ln[q]);
 
         // ---- receivers sitting on this node --------------------------------
-        const int r0 = a.rec_ptr[row], r1 = a.rec_ptr[row + 1];
+        const int r0 = ro(a.rec_ptr)[row], r1 = ro(a.rec_ptr)[row + 1];
         if (r0 < r1) {
             const double Un = (double)(dn[0] + dn[1] + dn[2]);
             const double Wn = (double)(dn[3] + dn[4] + dn[5]);
             for (int q = r0; q < r1; ++q) {
-                const size_t w = ((size_t)s * a.nsteps_max + t) * a.rec_num + a.rec_idx[q];
+                const size_t w = ((size_t)s * a.nsteps_max + t) * a.rec_num + ro(a.rec_idx)[q];
                 a.seis_u[w] = Un;
                 a.seis_w[w] = Wn;
             }
@@ -412,7 +423,7 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a, int slot
 {
     const int t = begin_step(a.ctrl, slot);
     constexpr unsigned int GS = SP >= 64 ? 64 : SP;
-    const unsigned int nact = (t < a.nsteps_max) ? (unsigned int)a.active[t] : 0u;
+    const unsigned int nact = (t < a.nsteps_max) ? (unsigned int)ro(a.active)[t] : 0u;
     const unsigned int groups = min(gridDim.y, (nact + GS - 1) / GS);
     const unsigned int nb = gridDim.x * groups;
     unsigned int bid = blockIdx.y * gridDim.x + blockIdx.x;
