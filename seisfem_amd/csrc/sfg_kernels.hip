@@ -191,9 +191,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             const size_t o0 = node_base(row);
 #pragma unroll
             for (int q = 0; q < 6; ++q) pre_acc[q] = ld_once<NT>(in + (U1TT + q) * FS + o0);
-            // (variant 4 reads its Kahan terms after the gathers: the 6 registers
-            // they would hold keep it at 4 waves per SIMD with one row per wave)
-            if constexpr (KAHAN && V != 4) {
+            if constexpr (KAHAN) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) pre_c[q] = ld_once<NT>(cin + q * FS + co);
             }
@@ -385,7 +383,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             const T d = cur[U1 + q], v = cur[U1T + q], acc = cur[U1TT + q];
             if constexpr (KAHAN) {
                 const T inc = v * dt + (ca * acc + al * an[q]) * dt * dt;
-                const T y = inc - ((PRE && V != 4) ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
+                const T y = inc - (PRE ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
                 const T sum = d + y;
                 st_once<NT>(cout + q * FS + co, (sum - d) - y);
                 dn[q] = sum;
