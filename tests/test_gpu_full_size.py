@@ -1,0 +1,76 @@
+"""Full-size configuration 1 (Q4 200x200, 40 401 nodes, 1998 steps) on the GPU,
+checked through properties that hold at any size:
+
+* linearity: doubling the source amplitude doubles every FP64 seismogram
+  sample exactly (every operation of the loop is linear in the state and a
+  factor 2 commutes with rounding), except where values are subnormal;
+* mirror symmetry: homogeneous medium, source on the mesh's mirror line y =
+  1000 m, receivers symmetric about it: u even and w odd in y up to rounding
+  (the reference's force has a -3.5e-15 horizontal part, cos(90 deg) with its
+  truncated pi);
+* the FP32 + Kahan loop stays within 1e-4 relative L2 of the FP64 loop (u and
+  w concatenated), the accuracy bar of the batched forms.
+"""
+import numpy as np
+import pytest
+
+from seisfem_amd import gpu, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print("[gpu-case]", msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    case = workloads.config1()
+    prob = case["problem"]
+    op = gpu.Operator(prob["node_num"], prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    yield case, op, md
+    md.close()
+    op.close()
+
+
+def _run(cfg, precision, amplitude=1.0e10):
+    case, op, md = cfg
+    steps = case["steps_exec"]
+    run = gpu.Run(op, md, case["src"][:1], steps, case["rec"], case["dt"], case["f0"], case["t0"],
+                  amplitude=amplitude, precision=precision)
+    run.advance(steps)
+    u, w = run.seismograms()
+    run.close()
+    return u[0], w[0]
+
+
+def test_full_size_linearity_in_the_amplitude(cfg1):
+    say("config 1 full size, FP64, amplitude 1e10 and 2e10")
+    u1, w1 = _run(cfg1, gpu.FP64, 1.0e10)
+    u2, w2 = _run(cfg1, gpu.FP64, 2.0e10)
+    assert np.max(np.abs(u1)) > 1e-3
+    for a, b in ((u1, u2), (w1, w2)):
+        normal = np.abs(a) > 1e-290
+        assert np.array_equal(2.0 * a[normal], b[normal])
+        assert np.max(np.abs(2.0 * a[~normal] - b[~normal]), initial=0.0) < 1e-290
+
+
+def test_full_size_mirror_symmetry(cfg1):
+    case = cfg1[0]
+    rec_y = case["problem"]["y"][case["rec"]]
+    assert np.allclose(rec_y + rec_y[::-1], 2000.0)
+    say("config 1 full size, FP64, mirror symmetry about y = 1000 m")
+    u, w = _run(cfg1, gpu.FP64)
+    su = np.max(np.abs(u))
+    assert np.max(np.abs(u - u[:, ::-1])) < 1e-9 * su
+    assert np.max(np.abs(w + w[:, ::-1])) < 1e-9 * su
+
+
+def test_full_size_fp32_kahan_within_1e4_of_fp64(cfg1):
+    say("config 1 full size, FP32+Kahan vs FP64")
+    u64, w64 = _run(cfg1, gpu.FP64)
+    u32, w32 = _run(cfg1, gpu.FP32_KAHAN)
+    a = np.concatenate([u32.ravel(), w32.ravel()])
+    b = np.concatenate([u64.ravel(), w64.ravel()])
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-4
