@@ -33,11 +33,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
 
 
 def measured_traffic(workload_key):
+    """(bytes per launch, launch shape it was measured with) or (None, None)."""
     try:
         with open(TRAFFIC_FILE) as f:
-            return json.load(f).get(workload_key)
+            d = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, None
+    return d.get(workload_key), d.get("_about", {}).get(workload_key, {}).get("launch")
 
 
 def log(msg):
@@ -255,6 +257,9 @@ def main():
                          f"reference issues them, OpenMP over rows"}
         log(f"cpu baseline: {cpu['value'] / 1e6:.1f} M DOF-steps/s on {threads} threads ({tc:.1f}s)")
 
+    # L2-miss bytes per step-kernel launch from the committed rocprofv3 runs, and the
+    # launch shape ("shots_per_lane_group,block,variant,xcd") they were measured with
+    traffic, traffic_launch = measured_traffic(f"{args.precision}_{args.shots}x_q4_{args.nelem}")
     out = {
         "metric": "DOF-steps/s + %HBM peak, batched FP32 vs plain FP64, at <=1e-4 rel-L2 seismograms",
         "value": value,
@@ -275,7 +280,8 @@ def main():
                    "tuned_fp64": autotune.describe(best64)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"{args.precision}_{args.shots}x_q4_{args.nelem}"),
+                     "traffic": traffic,
+                     "traffic_launch": traffic_launch,
                      "kernel": "step_kernel (fused SpMM + M-PML RHS + lumped solve + Newmark)",
                      "ms_per_launch": ms_step_dev, "bytes_per_launch": bytes_launch,
                      "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs,

@@ -590,13 +590,25 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
             const double wu = d->src_count ? d->src_weight[2 * q] : su;
             const double ww = d->src_count ? d->src_weight[2 * q + 1] : sw;
             if (node < 0) continue; // no source for this shot
-            bool merged = false;
-            for (auto &e2 : ent)
-                if (e2.node == node && e2.shot == s) { e2.wu += wu; e2.ww += ww; merged = true; }
-            if (!merged) ent.push_back({node, s, wu, ww});
+            ent.push_back({node, s, wu, ww});
         }
     }
-    std::stable_sort(ent.begin(), ent.end(), [](const SrcEntry &x, const SrcEntry &y) { return x.node < y.node; });
+    // entries grouped by (node, shot); repeated (node, shot) pairs are summed in input order
+    std::stable_sort(ent.begin(), ent.end(), [](const SrcEntry &x, const SrcEntry &y) {
+        return x.node != y.node ? x.node < y.node : x.shot < y.shot;
+    });
+    {
+        size_t m = 0;
+        for (size_t q = 0; q < ent.size(); ++q) {
+            if (m > 0 && ent[m - 1].node == ent[q].node && ent[m - 1].shot == ent[q].shot) {
+                ent[m - 1].wu += ent[q].wu;
+                ent[m - 1].ww += ent[q].ww;
+            } else {
+                ent[m++] = ent[q];
+            }
+        }
+        ent.resize(m);
+    }
     std::vector<int> sptr(n + 1, 0), sshot(std::max<size_t>(ent.size(), 1), 0);
     std::vector<double> swt(2 * std::max<size_t>(ent.size(), 1), 0.0);
     for (size_t q = 0; q < ent.size(); ++q) {
