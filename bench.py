@@ -93,9 +93,12 @@ def main():
     K = int(nsteps.max())
     log(f"setup {time.time() - t_setup:.1f}s: Q4 {args.nelem}x{args.nelem}, N={N}, shots={args.shots}, "
         f"steps {nsteps.min()}..{K}")
+    setup_s = time.time() - t_setup
     prec = gpu.PRECISION[args.precision]
+    t_up = time.time()
     op = gpu.Operator(N, prob["csr"])
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    upload_s = time.time() - t_up
     base = dict(dt=case["dt"], f0=case["f0"], t0=case["t0"])
     # launch shape chosen by measurement on this problem (untimed)
     coords = (prob["x"], prob["y"])
@@ -251,7 +254,14 @@ def main():
         t_cal = cpu_run(20)
         ksteps = int(min(K, max(20, args.cpu_seconds * 20 / max(t_cal, 1e-6))))
         tc = cpu_run(ksteps)
+        model = "unknown CPU"
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
         cpu = {"value": 2.0 * N * ksteps / tc, "unit": "DOF-steps/s", "cores": threads, "kind": "port",
+               "cpu_model": model,
                "sample": f"FP64 single shot, the first {ksteps} steps of shot 0 on the same Q4 {args.nelem}x"
                          f"{args.nelem} mesh ({tc:.1f}s wall incl. setup), 44 csr_matvec calls per step as the "
                          f"reference issues them, OpenMP over rows"}
@@ -276,7 +286,8 @@ def main():
         "config": {"workload": f"{args.shots} shots x Q4 {args.nelem}x{args.nelem} (dx=10 m), "
                                f"uneven steps U[{int(nsteps.min())},{K}], {args.precision}",
                    "nodes": N, "dof_per_shot": 2 * N, "shots": args.shots, "precision": args.precision,
-                   "parallelism": "1 GPU", "tuned": autotune.describe(best),
+                   "parallelism": "1 GPU", "host_setup_s": setup_s, "upload_s": upload_s,
+                   "tuned": autotune.describe(best),
                    "tuned_fp64": autotune.describe(best64)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
