@@ -1045,6 +1045,7 @@ extern "C" int sfg_estimate_dt(const sfg_operator *op, const sfg_medium *md, int
 struct sfg_multi {
     std::vector<sfg_run *> runs;
     int prec = 0, block = 256, ntiles = 0, max_steps = 0, guard_every = 50, executed = 0;
+    int xcd_remap = 1;          // the first run's flag: contiguous tile ranges per XCD, or round-robin
     Ctrl *ctrl = nullptr;       // shared step counter
     void *args = nullptr;       // device StepArgs<T>[nruns]
     int *tile_start = nullptr;  // device [nruns + 1]
@@ -1070,7 +1071,7 @@ static void multi_free(sfg_multi *m)
 
 static int multi_launch_step(sfg_multi *m, int slot)
 {
-    HIPCHK(launch_multi_step(m->prec, m->args, m->tile_start, (int)m->runs.size(), m->ntiles, m->block, 1, slot,
+    HIPCHK(launch_multi_step(m->prec, m->args, m->tile_start, (int)m->runs.size(), m->ntiles, m->block, m->xcd_remap, slot,
                              m->stream));
     return SFG_OK;
 }
@@ -1105,6 +1106,7 @@ extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns
     m->block = r0->block;
     m->max_steps = r0->max_steps;
     m->guard_every = r0->guard_every;
+    m->xcd_remap = r0->prec == 0 ? r0->a64.xcd_remap : r0->a32.xcd_remap;
     std::vector<int> ts(nruns + 1, 0);
     for (int k = 0; k < nruns; ++k) {
         m->runs.push_back(runs[k]);
