@@ -188,7 +188,8 @@ void sfg_run_destroy(sfg_run *run);
  * the launches of guard_every steps are captured in one hipGraph.  The runs
  * must share precision, max_steps (use per-shot nsteps for uneven lengths),
  * block size and guard interval, have lane groups of 1, 32 or 64 shots, and not
- * have stepped yet.  While grouped, a run is advanced only through the group;
+ * have stepped yet; the group's launch follows the first run's xcd_remap.
+ * While grouped, a run is advanced only through the group;
  * its seismograms and snapshots are read as usual.  No counterpart in the
  * reference, which runs one model per process. */
 typedef struct sfg_multi sfg_multi;
