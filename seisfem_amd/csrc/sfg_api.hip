@@ -387,6 +387,8 @@ struct sfg_run {
     std::vector<int> perm;      // internal shot -> user shot
     std::vector<int> steps_int; // executed steps per internal shot
     void *state = nullptr, *comp = nullptr;
+    void *uw = nullptr;         // FP32 forms: [2][2][n][S] U and W sums of the state
+    int *pml = nullptr;         // FP32 forms: rows with a nonzero M-PML profile
     Ctrl *ctrl = nullptr;
     int *active = nullptr, *src = nullptr, *rec_ptr = nullptr, *rec_idx = nullptr;
     int *order = nullptr;       // row_order on the device (or null)
@@ -414,6 +416,8 @@ static void run_free(sfg_run *r)
     if (r->stream) (void)hipStreamDestroy(r->stream);
     dfree(r->state);
     dfree(r->comp);
+    dfree(r->uw);
+    dfree(r->pml);
     dfree(r->ctrl);
     dfree(r->active);
     dfree(r->order);
@@ -706,7 +710,9 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     int rc = SFG_OK;
     hipError_t e = hipSuccess;
     if ((rc = dalloc((char **)&r->state, 2 * NFIELD * F * w)) ||
-        (r->prec == 2 && (rc = dalloc((char **)&r->comp, 2 * 6 * F * w))) || (rc = dalloc(&r->ctrl, 1)) ||
+        (r->prec == 2 && (rc = dalloc((char **)&r->comp, 2 * 6 * F * w))) ||
+        (r->prec != 0 && ((rc = dalloc((char **)&r->uw, 2 * 2 * F * w)) || (rc = dalloc(&r->pml, n)))) ||
+        (rc = dalloc(&r->ctrl, 1)) ||
         (rc = dalloc(&r->active, d->max_steps)) || (rc = dalloc(&r->src, n + 1)) ||
         (rc = dalloc(&r->src_shot, sshot.size())) || (rc = dalloc(&r->src_wt, swt.size())) ||
         (rc = dalloc(&r->rec_ptr, n + 1)) || (rc = dalloc(&r->rec_idx, ridx.size())) ||
@@ -723,6 +729,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (e == hipSuccess) e = hipEventCreate(&r->ev1);
     if (e == hipSuccess) e = launch_fill_zero(r->state, 2 * NFIELD * F * w, r->stream);
     if (e == hipSuccess && r->comp) e = launch_fill_zero(r->comp, 2 * 6 * F * w, r->stream);
+    if (e == hipSuccess && r->uw) e = launch_fill_zero(r->uw, 2 * 2 * F * w, r->stream);
     if (e == hipSuccess) e = launch_fill_zero(r->ctrl, sizeof(Ctrl), r->stream);
     if (e == hipSuccess) e = launch_fill_zero(r->seis_u, seis_n * 8, r->stream);
     if (e == hipSuccess) e = launch_fill_zero(r->seis_w, seis_n * 8, r->stream);
@@ -774,6 +781,17 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         }
     }
 
+    if (r->pml) {
+        // rows outside every damping layer (for all shots) take the short gather
+        e = launch_pml_rows(r->prec, r->cps ? r->coefp : (const void *)md->coef32, n, r->cps ? r->S : 1, r->pml,
+                            r->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+        if (e != hipSuccess) {
+            run_free(r);
+            return fail(SFG_E_HIP, "run setup (M-PML rows): %s", hipGetErrorString(e));
+        }
+    }
+
     // kernel arguments (identical for every step: the step index lives on the device)
     const double alpha = 1.0, delta = 1.5; // elastic_wave.c:97
     auto fill = [&](auto &a, auto *state, auto *comp, auto *const vals, auto *coef, auto cast) {
@@ -801,6 +819,8 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.flag = r->cps ? r->flagp : md->flag;
         a.state = state;
         a.comp = comp;
+        a.uw = (decltype(comp))r->uw;
+        a.pml = r->pml;
         a.fstride = F;
         a.active = r->active;
         a.src_ptr = r->src;

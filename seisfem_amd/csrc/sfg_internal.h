@@ -91,6 +91,13 @@ struct StepArgs {
     const int *flag;        // Dirichlet flags (exact mode), may be null otherwise
     T *state;               // [2][NFIELD][n][S]
     T *comp;                // [2][6][n][S] Kahan compensation of U1..W3 (or null)
+    // FP32 forms: U = u1 + u2 + u3 and W = w1 + w2 + w3 of each state buffer,
+    // stored by the step that produces it, [2][2][n][S] in the state's layout
+    // with 2 fields (null in exact mode).  pml[i] = 1 when one of the six M-PML
+    // profiles of row i is nonzero for some shot; a row with pml[i] = 0 gathers
+    // only U, W and the 8 memory fields of its neighbours (null in exact mode).
+    T *uw;
+    const int *pml;
     size_t fstride;         // n * S
     const int *active;      // active[t] = shots that execute step t (shots sorted)
     // sources grouped by node: entries src_ptr[i]..src_ptr[i+1]-1 of node i hold
@@ -131,6 +138,9 @@ hipError_t launch_to_f32(const double *src, float *dst, size_t n, hipStream_t st
 // (formed in FP64) or 0 on clamped / zero-mass nodes
 hipError_t launch_medium_f32(int n, const double *coef64, const int *flag, float *coef32, hipStream_t st);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t st);
+// pml[i] = 1 when coefficient DX..DYYP of row i is nonzero for one of the ns
+// shots: coef is [NCOEF][n][ns] (ns = 1: one medium for every shot)
+hipError_t launch_pml_rows(int precision, const void *coef, int n, int ns, int *pml, hipStream_t st);
 // dst[c][i][s] = srcs[min(s, nsrc-1)][c*n + i] for c < ncoef, i < n, s < S; elem = 4 or 8 bytes
 hipError_t launch_pack_per_shot(const void *const *srcs_dev, int nsrc, int ncoef, int n, int S, int elem,
                                 void *dst, hipStream_t st);

@@ -100,6 +100,12 @@ __device__ __forceinline__ int begin_step(Ctrl *ctrl, int slot)
 // 6 (node-major layouts only) stages the 20 gathered fields of the block's halo
 // nodes into LDS once, and every row of the block gathers from there: each
 // node is read from L2 once per block instead of once per row that needs it.
+// FP32 forms: the products of U1..W3 and U1t..W3t with M, and of U and W with
+// S5 and S6, enter a row's equations only multiplied by its M-PML profiles.
+// A row whose six profiles are all zero (outside the damping layers) leaves
+// them at 0 and gathers only U, W (stored by the producing step) and Lx1..Ly4:
+// 10 values per neighbour instead of 20.  Each skipped term is then an exact
+// zero in the same expressions, so the results equal those of the full walk.
 template <class T, int PREC, int SP, bool CPS, int V>
 __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, int by)
 {
@@ -111,6 +117,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
     constexpr bool PRE = V >= 3;                 // own-row accelerations / Kahan terms first
     constexpr bool CAP = V >= 3 && !LDSG;        // own-row fields from the diagonal gather
     constexpr int UN = V == 5 ? 4 : V == 4 ? 3 : 2;  // nonzeros per trip of the gather loop
+    constexpr bool WUW = !EXACT;                 // store U and W of the new state
+    constexpr bool SKIP = !EXACT && !LDSG;       // interior rows gather 10 values per neighbour
     constexpr int RPW = (SP >= 64) ? 1 : 64 / SP;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -177,6 +185,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             if constexpr (NM) return ((size_t)node * (S / GS) + by) * (NFIELD * GS) + ls;
             else return (size_t)node * S + s;
         };
+        // U and W: 2 fields in the same layout as the state
+        auto uw_base = [&](int node) __attribute__((always_inline)) -> size_t {
+            if constexpr (NM) return ((size_t)node * (S / GS) + by) * (2 * GS) + ls;
+            else return (size_t)node * S + s;
+        };
+        const T *uwin = WUW ? a.uw + (size_t)(t & 1) * 2 * F : nullptr;
+        T *uwout = WUW ? a.uw + (size_t)((t + 1) & 1) * 2 * F : nullptr;
+        bool interior = false;
+        if constexpr (SKIP) interior = ro(a.pml)[row] == 0;
         // Kahan terms: 6 fields in the same layout as the state
         const size_t C6 = (size_t)6 * F;
         const T *cin = KAHAN ? a.comp + (size_t)(t & 1) * C6 : nullptr;
@@ -266,17 +283,61 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             mw3t = mac<EXACT>(mw3t, am, w3t); mw3 = mac<EXACT>(mw3, am, w3);
             mly3 = mac<EXACT>(mly3, am, ly3); mly4 = mac<EXACT>(mly4, am, ly4);
         };
-        if constexpr (PACKED) {
-            // UN nonzeros per trip: their gathers are in flight together; each
-            // sum still runs in column order
-            int k = k0;
-            for (; k + UN - 1 < k1; k += UN) {
-#pragma unroll
-                for (int u = 0; u < UN; ++u) nonzero(k + u);
+        // interior row: U, W and the memory fields of neighbour j
+        auto nonzero_in = [&](int k) __attribute__((always_inline)) {
+            const int j = ro(a.ci)[k];
+            T am, a1, a2, a3, a4;
+            if constexpr (PACKED) {
+                const auto v = ro(a.Ap) + (size_t)k * 8;
+                am = v[0]; a1 = v[1]; a2 = v[2]; a3 = v[3]; a4 = v[4];
+            } else {
+                am = ro(a.A[0])[k];
+                a1 = ro(a.A[1])[k]; a2 = ro(a.A[2])[k]; a3 = ro(a.A[3])[k]; a4 = ro(a.A[4])[k];
             }
-            for (; k < k1; ++k) nonzero(k);
+            const T *x = in + node_base(j);
+            const T *xu = uwin + uw_base(j);
+            const T U = xu[0], W = xu[FS];
+            const T lx1 = x[LX1 * FS], lx2 = x[LX2 * FS], lx3 = x[LX3 * FS], lx4 = x[LX4 * FS];
+            const T ly1 = x[LY1 * FS], ly2 = x[LY2 * FS], ly3 = x[LY3 * FS], ly4 = x[LY4 * FS];
+            if constexpr (CAP) {
+                if (j == row) {
+                    cur[LX1] = lx1; cur[LX2] = lx2; cur[LX3] = lx3; cur[LX4] = lx4;
+                    cur[LY1] = ly1; cur[LY2] = ly2; cur[LY3] = ly3; cur[LY4] = ly4;
+                    got = true;
+                }
+            }
+            s1u = mac<EXACT>(s1u, a1, U); s1w = mac<EXACT>(s1w, a1, W);
+            s2u = mac<EXACT>(s2u, a2, U); s2w = mac<EXACT>(s2w, a2, W);
+            s3u = mac<EXACT>(s3u, a3, U); s3w = mac<EXACT>(s3w, a3, W);
+            s4u = mac<EXACT>(s4u, a4, U); s4w = mac<EXACT>(s4w, a4, W);
+            mlx1 = mac<EXACT>(mlx1, am, lx1); mlx2 = mac<EXACT>(mlx2, am, lx2);
+            mlx3 = mac<EXACT>(mlx3, am, lx3); mlx4 = mac<EXACT>(mlx4, am, lx4);
+            mly1 = mac<EXACT>(mly1, am, ly1); mly2 = mac<EXACT>(mly2, am, ly2);
+            mly3 = mac<EXACT>(mly3, am, ly3); mly4 = mac<EXACT>(mly4, am, ly4);
+        };
+        auto walk = [&](auto &&nz) __attribute__((always_inline)) {
+            if constexpr (PACKED) {
+                // UN nonzeros per trip: their gathers are in flight together; each
+                // sum still runs in column order
+                int k = k0;
+                for (; k + UN - 1 < k1; k += UN) {
+#pragma unroll
+                    for (int u = 0; u < UN; ++u) nz(k + u);
+                }
+                for (; k < k1; ++k) nz(k);
+            } else {
+                for (int k = k0; k < k1; ++k) nz(k);
+            }
+        };
+        if (interior) {
+            // the row's own displacements and velocities are not gathered here
+            if constexpr (PRE) {
+#pragma unroll
+                for (int f = U1; f <= W3T; ++f) cur[f] = ld_once<NT>(in + f * FS + o);
+            }
+            walk(nonzero_in);
         } else {
-            for (int k = k0; k < k1; ++k) nonzero(k);
+            walk(nonzero);
         }
 
         // ---- per-node coefficients --------------------------------------
@@ -333,6 +394,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
 #pragma unroll
             for (int f = 0; f < NFIELD; ++f)
                 if (f < U1TT || f > W3TT) cur[f] = x[gathered_slot(f) * GS];
+        } else if (interior) {
+            if constexpr (!PRE) {
+#pragma unroll
+                for (int f = U1; f <= W3T; ++f) cur[f] = in[f * FS + o];
+            }
+            if (!CAP || !got) {
+#pragma unroll
+                for (int f = LX1; f < NFIELD; ++f) cur[f] = in[f * FS + o];
+            }
         } else if (!CAP || !got) {
             // (every row of an assembled operator holds its diagonal, so with
             // CAP this runs only for a pattern without one)
@@ -396,6 +466,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) st_once<NT>(out + (LX1 + q) * FS + o, ln[q]);
+        if constexpr (WUW) {
+            // the sums every neighbour forms from the split fields, in the same order
+            const size_t ob = uw_base(row);
+            st_once<NT>(uwout + ob, dn[0] + dn[1] + dn[2]);
+            st_once<NT>(uwout + ob + FS, dn[3] + dn[4] + dn[5]);
+        }
 
         // ---- receivers sitting on this node --------------------------------
         const int r0 = ro(a.rec_ptr)[row], r1 = ro(a.rec_ptr)[row + 1];
@@ -454,21 +530,22 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
     const int local = (int)bid - tile_start[r];
     const int bx = local % a.nbx, by = local / a.nbx;
     // Kernel variants of one launch: separate value arrays (0), packed values
-    // (1; also runs 2, 3 and 6), and for 32- and 64-shot groups the unroll-3
-    // form with own-row data first (4; also runs 5).  Single-shot rows keep 0/1,
-    // whose register count leaves room for more waves.
+    // (1; also runs 2 and 6), and for 32- and 64-shot groups the form with
+    // own-row data first (3; also runs 4 and 5).  The launch's register count is
+    // that of its largest form, so the deeper unrolls are left out: they would
+    // cost every run a wave per SIMD.
     const int v = a.variant;
-    const int w = v == 0 ? 0 : (v == 4 || v == 5) ? 4 : 1;
+    const int w = v == 0 ? 0 : (v >= 3 && v <= 5) ? 3 : 1;
 #define SFG_T(SPV, C, VV) step_tile<T, PREC, SPV, C, VV>(a, t, bx, by)
     if (a.sp == 1) {
         if (a.cps) { if (w) SFG_T(1, true, 1); else SFG_T(1, true, 0); }
         else { if (w) SFG_T(1, false, 1); else SFG_T(1, false, 0); }
     } else if (a.sp == 32) {
-        if (a.cps) { if (w == 4) SFG_T(32, true, 4); else if (w) SFG_T(32, true, 1); else SFG_T(32, true, 0); }
-        else { if (w == 4) SFG_T(32, false, 4); else if (w) SFG_T(32, false, 1); else SFG_T(32, false, 0); }
+        if (a.cps) { if (w == 3) SFG_T(32, true, 3); else if (w) SFG_T(32, true, 1); else SFG_T(32, true, 0); }
+        else { if (w == 3) SFG_T(32, false, 3); else if (w) SFG_T(32, false, 1); else SFG_T(32, false, 0); }
     } else {
-        if (a.cps) { if (w == 4) SFG_T(64, true, 4); else if (w) SFG_T(64, true, 1); else SFG_T(64, true, 0); }
-        else { if (w == 4) SFG_T(64, false, 4); else if (w) SFG_T(64, false, 1); else SFG_T(64, false, 0); }
+        if (a.cps) { if (w == 3) SFG_T(64, true, 3); else if (w) SFG_T(64, true, 1); else SFG_T(64, true, 0); }
+        else { if (w == 3) SFG_T(64, false, 3); else if (w) SFG_T(64, false, 1); else SFG_T(64, false, 0); }
     }
 #undef SFG_T
 }
@@ -781,6 +858,29 @@ __global__ void fill_zero_kernel(uint32_t *p, size_t nwords)
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nwords;
          i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
+}
+
+template <class T>
+__global__ void pml_rows_kernel(const T *coef, int n, int ns, int *pml)
+{
+    const size_t cs = (size_t)n * ns;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int any = 0;
+        for (int c = DX; c <= DYYP; ++c)
+            for (int s = 0; s < ns; ++s) any |= coef[c * cs + (size_t)i * ns + s] != T(0);
+        pml[i] = any;
+    }
+}
+
+hipError_t launch_pml_rows(int precision, const void *coef, int n, int ns, int *pml, hipStream_t st)
+{
+    const int blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+    if (n <= 0) return hipSuccess;
+    if (precision == 0)
+        hipLaunchKernelGGL(pml_rows_kernel<double>, dim3(blocks), dim3(256), 0, st, (const double *)coef, n, ns, pml);
+    else
+        hipLaunchKernelGGL(pml_rows_kernel<float>, dim3(blocks), dim3(256), 0, st, (const float *)coef, n, ns, pml);
+    return hipGetLastError();
 }
 
 __global__ void medium_f32_kernel(int n, const double *c64, const int *flag, float *c32)

@@ -188,8 +188,45 @@ def test_kernel_variants_agree_bit_for_bit(sp, prec, block):
         res[-1] = res[-1] + (U, W)
         run.close()
     assert np.max(np.abs(res[0][0])) > 0
+    for v, r in enumerate(res[1:], start=2):
+        if v >= 7 and prec != gpu.FP64:
+            # the LDS variant always walks all 20 fields; the FP32 forms of the
+            # others skip terms that are exact zeros outside the damping layers,
+            # which can change the sign of a zero result, never its value
+            assert all(np.array_equal(x, y) for x, y in zip(r, res[0]))
+        else:
+            assert all(bits_equal(x, y) for x, y in zip(r, res[0]))
+    md.close()
+    op.close()
+
+
+@pytest.mark.parametrize("prec", [gpu.FP32, gpu.FP32_KAHAN])
+def test_fp32_interior_rows_short_gather_equals_full_walk(prec):
+    """FP32 forms: a row outside every damping layer gathers U, W and the
+    memory fields only (10 values per neighbour instead of 20).  The terms it
+    skips enter its equations multiplied by M-PML profiles that are exactly
+    zero, so seismograms at every node and the final wavefield equal those of
+    the full walk (the LDS-staged variant 7, which always gathers all 20) for
+    lane groups of 1, 32 and 64 shots."""
+    case = _case(host.Q4, 12, 50, nshots=40)
+    prob = case["problem"]
+    n = prob["node_num"]
+    pml_rows = np.any(np.asarray(prob["mpml"]) != 0, axis=0)
+    assert 0.2 < pml_rows.mean() < 0.9, "the mesh needs rows of both kinds"
+    say(f"short gather vs full walk: Q4 12x12 shots=40 precision={prec}, {pml_rows.mean():.2f} M-PML rows")
+    op = gpu.Operator(n, prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    res = []
+    for sp, v, block in [(64, 7, 64), (64, 5, 64), (64, 1, 256), (32, 6, 128), (1, 1, 64)]:
+        run = gpu.Run(op, md, case["src"], 50, np.arange(n, dtype=np.int32), case["dt"], case["f0"], case["t0"],
+                      precision=prec, shots_per_lane_group=sp, guard_every=25, kernel_variant=v, block=block)
+        run.advance(50)
+        U, W = run.snapshot(11)
+        res.append(run.seismograms() + (U, W))
+        run.close()
+    assert np.max(np.abs(res[0][0])) > 0
     for r in res[1:]:
-        assert all(bits_equal(x, y) for x, y in zip(r, res[0]))
+        assert all(np.array_equal(x, y) for x, y in zip(r, res[0]))
     md.close()
     op.close()
 
