@@ -783,8 +783,8 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
 
     if (r->pml) {
         // rows outside every damping layer (for all shots) take the short gather
-        e = launch_pml_rows(r->prec, r->cps ? r->coefp : (const void *)md->coef32, n, r->cps ? r->S : 1, r->pml,
-                            r->stream);
+        e = launch_pml_rows(r->prec, r->cps ? r->coefp : (const void *)md->coef32, n, r->cps ? r->S : 1, op->rp,
+                            op->ci, r->pml, r->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
         if (e != hipSuccess) {
             run_free(r);
@@ -793,7 +793,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     }
 
     // kernel arguments (identical for every step: the step index lives on the device)
-    const double alpha = 1.0, delta = 1.5; // elastic_wave.c:97
+    const double alpha = NEWMARK_ALPHA, delta = NEWMARK_DELTA;
     auto fill = [&](auto &a, auto *state, auto *comp, auto *const vals, auto *coef, auto cast) {
         a.n = n;
         a.S = r->S;

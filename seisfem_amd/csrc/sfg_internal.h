@@ -26,6 +26,14 @@ enum Coef : int {
     C11 = 0, C13, C33, C44, DX, DY, DXX, DYY, DXXP, DYYP, MASS, MROW, NCOEF
 };
 
+// FP32 forms keep p = v + (0.5 - alpha) dt a in the acceleration slots: with
+// the reference's alpha = 1, delta = 1.5 (so 1 - delta = 0.5 - alpha) the
+// Newmark update becomes d' = d + p dt + alpha a' dt^2, p' = p + a' dt and
+// v' = p + delta a' dt, so the acceleration is not kept and the velocity only
+// where an M-PML row gathers it.
+constexpr double NEWMARK_ALPHA = 1.0, NEWMARK_DELTA = 1.5;  // elastic_wave.c:97
+static_assert(1.0 - NEWMARK_DELTA == 0.5 - NEWMARK_ALPHA, "p-form needs delta = alpha + 0.5");
+
 // Device-side control block: the step counter lets one captured graph replay
 // any chunk of steps.  A step kernel launched with slot p reads its step t from
 // step[p] and one thread of it stores t + 1 into step[p ^ 1], which the next
@@ -93,9 +101,10 @@ struct StepArgs {
     T *comp;                // [2][6][n][S] Kahan compensation of U1..W3 (or null)
     // FP32 forms: U = u1 + u2 + u3 and W = w1 + w2 + w3 of each state buffer,
     // stored by the step that produces it, [2][2][n][S] in the state's layout
-    // with 2 fields (null in exact mode).  pml[i] = 1 when one of the six M-PML
-    // profiles of row i is nonzero for some shot; a row with pml[i] = 0 gathers
-    // only U, W and the 8 memory fields of its neighbours (null in exact mode).
+    // with 2 fields (null in exact mode).  pml[i] = 2 when one of the six M-PML
+    // profiles of row i is nonzero for some shot, 1 when row i is a column of
+    // such a row, else 0.  Rows below 2 gather only U, W and the 8 memory
+    // fields of their neighbours; rows at 0 keep no velocity (null in exact mode).
     T *uw;
     const int *pml;
     size_t fstride;         // n * S
@@ -138,9 +147,11 @@ hipError_t launch_to_f32(const double *src, float *dst, size_t n, hipStream_t st
 // (formed in FP64) or 0 on clamped / zero-mass nodes
 hipError_t launch_medium_f32(int n, const double *coef64, const int *flag, float *coef32, hipStream_t st);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t st);
-// pml[i] = 1 when coefficient DX..DYYP of row i is nonzero for one of the ns
-// shots: coef is [NCOEF][n][ns] (ns = 1: one medium for every shot)
-hipError_t launch_pml_rows(int precision, const void *coef, int n, int ns, int *pml, hipStream_t st);
+// pml[i] = 2 when coefficient DX..DYYP of row i is nonzero for one of the ns
+// shots (coef is [NCOEF][n][ns]; ns = 1: one medium for every shot), 1 when
+// row i is a column of such a row in the pattern rp/ci, else 0
+hipError_t launch_pml_rows(int precision, const void *coef, int n, int ns, const int *rp, const int *ci, int *pml,
+                           hipStream_t st);
 // dst[c][i][s] = srcs[min(s, nsrc-1)][c*n + i] for c < ncoef, i < n, s < S; elem = 4 or 8 bytes
 hipError_t launch_pack_per_shot(const void *const *srcs_dev, int nsrc, int ncoef, int n, int S, int elem,
                                 void *dst, hipStream_t st);

@@ -106,6 +106,9 @@ __device__ __forceinline__ int begin_step(Ctrl *ctrl, int slot)
 // them at 0 and gathers only U, W (stored by the producing step) and Lx1..Ly4:
 // 10 values per neighbour instead of 20.  Each skipped term is then an exact
 // zero in the same expressions, so the results equal those of the full walk.
+// The FP32 forms also update displacements with the p-form of the Newmark
+// step (NEWMARK_ALPHA): a row far from the damping layers reads and writes
+// displacements, p and memory fields only.
 template <class T, int PREC, int SP, bool CPS, int V>
 __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, int by)
 {
@@ -192,8 +195,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         };
         const T *uwin = WUW ? a.uw + (size_t)(t & 1) * 2 * F : nullptr;
         T *uwout = WUW ? a.uw + (size_t)((t + 1) & 1) * 2 * F : nullptr;
-        bool interior = false;
-        if constexpr (SKIP) interior = ro(a.pml)[row] == 0;
+        int pk = 2;                              // row kind: see StepArgs::pml
+        if constexpr (!EXACT) pk = ro(a.pml)[row];
+        const bool interior = SKIP && pk < 2;    // short gather
+        const bool keepv = EXACT || pk != 0;     // velocity stored
         // Kahan terms: 6 fields in the same layout as the state
         const size_t C6 = (size_t)6 * F;
         const T *cin = KAHAN ? a.comp + (size_t)(t & 1) * C6 : nullptr;
@@ -330,10 +335,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             }
         };
         if (interior) {
-            // the row's own displacements and velocities are not gathered here
+            // the row's own displacements are not gathered here
             if constexpr (PRE) {
 #pragma unroll
-                for (int f = U1; f <= W3T; ++f) cur[f] = ld_once<NT>(in + f * FS + o);
+                for (int f = U1; f <= W3; ++f) cur[f] = ld_once<NT>(in + f * FS + o);
             }
             walk(nonzero_in);
         } else {
@@ -397,7 +402,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         } else if (interior) {
             if constexpr (!PRE) {
 #pragma unroll
-                for (int f = U1; f <= W3T; ++f) cur[f] = in[f * FS + o];
+                for (int f = U1; f <= W3; ++f) cur[f] = in[f * FS + o];
             }
             if (!CAP || !got) {
 #pragma unroll
@@ -450,18 +455,27 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         T dn[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            const T d = cur[U1 + q], v = cur[U1T + q], acc = cur[U1TT + q];
-            if constexpr (KAHAN) {
-                const T inc = v * dt + (ca * acc + al * an[q]) * dt * dt;
-                const T y = inc - (PRE ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
-                const T sum = d + y;
-                st_once<NT>(cout + q * FS + co, (sum - d) - y);
-                dn[q] = sum;
-            } else {
+            const T d = cur[U1 + q];
+            if constexpr (EXACT) {
+                const T v = cur[U1T + q], acc = cur[U1TT + q];
                 dn[q] = d + v * dt + (ca * acc + al * an[q]) * dt * dt;
+                st_once<NT>(out + (U1T + q) * FS + o, v + (cd * acc + de * an[q]) * dt);
+                st_once<NT>(out + (U1TT + q) * FS + o, an[q]);
+            } else {
+                // p-form: the acceleration slot holds p = v + ca dt a
+                const T p = cur[U1TT + q];
+                if constexpr (KAHAN) {
+                    const T inc = p * dt + al * an[q] * dt * dt;
+                    const T y = inc - (PRE ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
+                    const T sum = d + y;
+                    st_once<NT>(cout + q * FS + co, (sum - d) - y);
+                    dn[q] = sum;
+                } else {
+                    dn[q] = d + p * dt + al * an[q] * dt * dt;
+                }
+                if (keepv) st_once<NT>(out + (U1T + q) * FS + o, p + de * an[q] * dt);
+                st_once<NT>(out + (U1TT + q) * FS + o, p + (de + ca) * an[q] * dt);
             }
-            st_once<NT>(out + (U1T + q) * FS + o, v + (cd * acc + de * an[q]) * dt);
-            st_once<NT>(out + (U1TT + q) * FS + o, an[q]);
             st_once<NT>(out + (U1 + q) * FS + o, dn[q]);
         }
 #pragma unroll
@@ -868,11 +882,25 @@ __global__ void pml_rows_kernel(const T *coef, int n, int ns, int *pml)
         int any = 0;
         for (int c = DX; c <= DYYP; ++c)
             for (int s = 0; s < ns; ++s) any |= coef[c * cs + (size_t)i * ns + s] != T(0);
-        pml[i] = any;
+        pml[i] = any ? 2 : 0;
     }
 }
 
-hipError_t launch_pml_rows(int precision, const void *coef, int n, int ns, int *pml, hipStream_t st)
+// Marks the columns of M-PML rows that are not M-PML rows themselves.  Threads
+// may store 1 to the same entry at once; a 2 is never overwritten.
+__global__ void pml_ring_kernel(int n, const int *rp, const int *ci, int *pml)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (pml[i] != 2) continue;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int j = ci[k];
+            if (pml[j] != 2) pml[j] = 1;
+        }
+    }
+}
+
+hipError_t launch_pml_rows(int precision, const void *coef, int n, int ns, const int *rp, const int *ci, int *pml,
+                           hipStream_t st)
 {
     const int blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
     if (n <= 0) return hipSuccess;
@@ -880,6 +908,9 @@ hipError_t launch_pml_rows(int precision, const void *coef, int n, int ns, int *
         hipLaunchKernelGGL(pml_rows_kernel<double>, dim3(blocks), dim3(256), 0, st, (const double *)coef, n, ns, pml);
     else
         hipLaunchKernelGGL(pml_rows_kernel<float>, dim3(blocks), dim3(256), 0, st, (const float *)coef, n, ns, pml);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(pml_ring_kernel, dim3(blocks), dim3(256), 0, st, n, rp, ci, pml);
     return hipGetLastError();
 }
 
