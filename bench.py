@@ -66,7 +66,8 @@ def main():
     ap.add_argument("--no-multi", action="store_true", help="skip config 4 part 2 (16 models in the same launch)")
     ap.add_argument("--quick", action="store_true", help="small mesh/steps for profiling runs")
     ap.add_argument("--launch", default="",
-                    help="pin the batched launch 'shots_per_lane_group,block,variant,xcd_remap[,hilbert]' (no tuning)")
+                    help="pin the batched launch 'shots_per_lane_group,block,variant,xcd[,hilbert]' (no tuning); "
+                         "xcd: 1 contiguous ranges, 0 round-robin, Q > 1 stripes of Q row blocks")
     ap.add_argument("--launch64", default="", help="pin the FP64 single-shot launch shape (same form)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -106,7 +107,8 @@ def main():
 
     def pinned(spec):
         v = [int(q) for q in spec.split(",")]
-        return dict(shots_per_lane_group=v[0], block=v[1], kernel_variant=v[2], xcd_remap=bool(v[3]),
+        return dict(shots_per_lane_group=v[0], block=v[1], kernel_variant=v[2],
+                    xcd_remap=v[3] if v[3] > 1 else bool(v[3]),
                     row_order=hilbert if len(v) > 4 and v[4] else None)
 
     if args.launch:

@@ -131,7 +131,9 @@ typedef struct {
                                  in a wavefront (64/SP rows per wave); 0 = chosen from nshots */
     int guard_every;        /* energy-guard interval in steps, 0 = 50 */
     int use_graph;          /* 1 = replay captured hipGraphs of guard_every steps */
-    int xcd_remap;          /* 0 = default (on): each XCD steps one contiguous row range; -1 = off */
+    int xcd_remap;          /* 0 = default (on): each XCD steps one contiguous row range; -1 = off
+                               (round-robin); Q > 0 = stripes: all XCDs sweep the rows together, XCD x
+                               taking row blocks x*Q .. x*Q+Q-1 of every 8*Q */
     /* Optional distributed sources (no counterpart in the reference, whose only
      * source is the point force at src_node): shot s adds wavelet * weight to
      * the u / w rows of src_count[s] nodes.  NULL src_count = point forces. */

@@ -1,7 +1,8 @@
 """Autotuner for the step kernel: picks the block size, the lane grouping
 (rows per wavefront vs shots per lane group), the kernel variant, the XCD
-block remap and the row order (node order or a Hilbert walk over the node
-coordinates) by timing short runs of the actual problem on the GPU.
+block map (contiguous ranges, round-robin, or stripes of Q row blocks) and the
+row order (node order or a Hilbert walk over the node coordinates) by timing
+short runs of the actual problem on the GPU.
 
 Each candidate runs ``steps`` steps (after ``steps`` untimed ones) as one
 captured graph with the energy guard at the end, timed with HIP events on the
@@ -28,7 +29,13 @@ def candidates(nshots):
     # variant 7 (LDS-staged gathers, lane groups of 16+ shots) is left out: it
     # measured about 2x slower than 5 on Q4 400x400 x 128 shots (the block's
     # staging and its compute do not overlap at 1-3 blocks per CU)
-    return list(itertools.product(sps, blocks, [1, 3, 4, 5, 6], [True, False], ["node", "hilbert"]))
+    variants = [1, 3, 4, 5, 6]
+    cands = list(itertools.product(sps, blocks, variants, [True, False], ["node", "hilbert"]))
+    if nshots > 1:
+        # XCD stripes of Q row blocks (node order): all XCDs sweep the rows together;
+        # Q near (row blocks per grid line) / 8 keeps vertical neighbours in one L2
+        cands += list(itertools.product(sps, blocks, variants, [6, 12, 25], ["node"]))
+    return cands
 
 
 def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None, coords=None):

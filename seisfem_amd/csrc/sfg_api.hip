@@ -505,6 +505,8 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (d->precision < 0 || d->precision > 2) return fail(SFG_E_ARG, "run: unknown precision %d", d->precision);
     if (d->kernel_variant < 0 || d->kernel_variant > 7)
         return fail(SFG_E_ARG, "run: unknown kernel_variant %d", d->kernel_variant);
+    if (d->xcd_remap < -1 || d->xcd_remap > (1 << 20))
+        return fail(SFG_E_ARG, "run: xcd_remap %d (0, -1 or a stripe of 1 .. 2^20 row blocks)", d->xcd_remap);
     if (d->nshots <= 0 || !d->src_node) return fail(SFG_E_ARG, "run: need >= 1 shot with a source node");
     if (nmedia != 1 && nmedia != d->nshots) return fail(SFG_E_ARG, "run: need one medium or one per shot");
     for (int k = 0; k < nmedia; ++k) {
@@ -799,7 +801,8 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.S = r->S;
         a.rows_per_block = rows_per_block;
         a.nsteps_max = d->max_steps;
-        a.xcd_remap = d->xcd_remap >= 0 ? 1 : 0;
+        a.xcd_remap = d->xcd_remap == 0 ? 1 : 0;
+        a.xcd_stripe = d->xcd_remap > 0 ? d->xcd_remap : 0;
         a.sp = r->SP;
         a.cps = r->cps ? 1 : 0;
         a.nbx = r->nbx;

@@ -83,6 +83,7 @@ struct StepArgs {
     int rows_per_block;
     int nsteps_max;
     int xcd_remap;          // 1: contiguous block ranges per XCD
+    int xcd_stripe;         // Q > 0: XCD x takes tiles x*Q .. x*Q+Q-1 of every 8*Q (overrides xcd_remap)
     int sp, cps, nbx;       // lane grouping, per-shot coefficients, row blocks (multi-run dispatch)
     const int *rp, *ci;     // union CSR pattern
     const int *order;       // row taken by logical row i (a permutation), or null = i

@@ -79,6 +79,19 @@ __device__ __forceinline__ unsigned int xcd_contiguous(unsigned int bid, unsigne
     return x * q + (x < r ? x : r) + l;
 }
 
+// Stripes: tiles are taken in runs of 8*Q; within a run XCD x (workgroup ids
+// x, x+8, ...) steps tiles x*Q .. x*Q+Q-1.  All XCDs then sweep the rows
+// together, each keeping one stripe of consecutive row blocks, so a row's
+// neighbours on the adjacent grid lines were fetched into the same XCD's L2.
+// A bijection on [0, nb): the ids past the last whole run map to themselves.
+__device__ __forceinline__ unsigned int xcd_striped(unsigned int bid, unsigned int nb, unsigned int q)
+{
+    const unsigned int span = 8u * q;
+    if (bid >= (nb / span) * span) return bid;
+    const unsigned int x = bid % 8, k = bid / 8;
+    return (k / q) * span + x * q + k % q;
+}
+
 // Step number of a launch with the given slot; the first thread of the grid
 // hands t + 1 to the next launch (the other slot).  One store per launch: a
 // per-block completion count on one address serialises tens of thousands of
@@ -518,7 +531,8 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a, int slot
     const unsigned int nb = gridDim.x * groups;
     unsigned int bid = blockIdx.y * gridDim.x + blockIdx.x;
     if (bid >= nb) return;
-    if (a.xcd_remap) bid = xcd_contiguous(bid, nb);
+    if (a.xcd_stripe > 0) bid = xcd_striped(bid, nb, (unsigned int)a.xcd_stripe);
+    else if (a.xcd_remap) bid = xcd_contiguous(bid, nb);
     step_tile<T, PREC, SP, CPS, V>(a, t, (int)(bid % gridDim.x), (int)(bid / gridDim.x));
 }
 

@@ -111,7 +111,8 @@ class Run:
         d.dt, d.f0, d.t0, d.amplitude, d.angle_deg = dt, f0, t0, amplitude, angle_deg
         d.block, d.shots_per_lane_group = block, shots_per_lane_group
         d.guard_every, d.use_graph = guard_every, int(bool(use_graph))
-        d.xcd_remap = 0 if xcd_remap else -1
+        # True: contiguous ranges per XCD; False: round-robin; an int Q > 1: stripes of Q row blocks
+        d.xcd_remap = xcd_remap if (type(xcd_remap) is int and xcd_remap > 1) else (0 if xcd_remap else -1)
         d.kernel_variant = int(kernel_variant)
         if row_order is not None:
             self._order = np.ascontiguousarray(row_order, np.int32)

@@ -139,20 +139,22 @@ def test_multishot_fp64_each_shot_bit_identical(nshots, sp):
 
 @pytest.mark.parametrize("nshots,sp,prec", [(1, 1, gpu.FP64), (70, 64, gpu.FP64), (45, 32, gpu.FP32_KAHAN)])
 def test_row_order_changes_nothing(nshots, sp, prec):
-    """Rows stepped in Hilbert order or in a random order give the same bits as
-    node order (each lane's arithmetic does not depend on which rows run together)."""
+    """Rows stepped in Hilbert order or in a random order, and tiles handed to
+    the XCDs round-robin or in stripes, give the same bits as node order with
+    contiguous XCD ranges (each lane's arithmetic does not depend on which rows
+    run together)."""
     case = _case(host.Q4, 14, 60, nshots=nshots)
     prob = case["problem"]
     n = prob["node_num"]
     say(f"row order: Q4 14x14 shots={nshots} sp={sp} precision={prec}")
     op = gpu.Operator(n, prob["csr"])
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
-    orders = [None, host.locality_order(prob["x"], prob["y"]),
-              np.random.default_rng(5).permutation(n).astype(np.int32)]
+    orders = [(None, True), (host.locality_order(prob["x"], prob["y"]), True),
+              (np.random.default_rng(5).permutation(n).astype(np.int32), True), (None, False), (None, 3)]
     res = []
-    for o in orders:
+    for o, xcd in orders:
         run = gpu.Run(op, md, case["src"], 60, np.arange(n, dtype=np.int32), case["dt"], case["f0"], case["t0"],
-                      precision=prec, shots_per_lane_group=sp, guard_every=20, row_order=o)
+                      precision=prec, shots_per_lane_group=sp, guard_every=20, row_order=o, xcd_remap=xcd)
         run.advance(60)
         res.append(run.seismograms())
         run.close()
