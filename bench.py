@@ -3,14 +3,16 @@
 Workload (default): 128 shots sharing one Q4 400x400 mesh (dx = 10 m,
 160 801 nodes, 321 602 DOF per shot), 6-layer isotropic model, M-PML 30,
 dt = 0.5 ms, executed steps drawn U[1500, 3000] per shot, FP32 state with a
-Kahan-compensated displacement update -- the batched multi-shot configuration.
-Inputs are resident in HBM before the timed region.
+Kahan-compensated displacement update whose compensation terms are kept as
+bfloat16 -- the batched multi-shot configuration.  Inputs are resident in HBM
+before the timed region.
 
   value        whole-job DOF-steps/s over the K timed steps (2 * N * active shots per step)
   roofline     the step kernel: algorithmic bytes per launch / average launch time
                (HIP events on the kernel's stream), against 8 TB/s
+  fp32_kahan_terms  the same job with the compensation terms in FP32
   fp64_single  the plain single-shot FP64 loop (bit-identical to the reference) on the same mesh
-  accuracy     max rel-L2 (u and w concatenated) of FP32+Kahan seismograms vs the FP64 loop
+  accuracy     max rel-L2 (u and w concatenated) of the FP32 seismograms vs the FP64 loop
   cpu_baseline the C restatement of the reference loop (oracle) on the host cores, bounded sample
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
@@ -57,7 +59,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nelem", type=int, default=400)
     ap.add_argument("--shots", type=int, default=128)
-    ap.add_argument("--precision", default="fp32_kahan", choices=["fp64", "fp32", "fp32_kahan"])
+    ap.add_argument("--precision", default="fp32_kahan16", choices=["fp64", "fp32", "fp32_kahan", "fp32_kahan16"],
+                    help="fp32_kahan16: FP32 with the Kahan terms kept as bfloat16 (the default); fp32_kahan: "
+                         "the terms in FP32")
+    ap.add_argument("--no-kahan32", action="store_true",
+                    help="skip the second timed job with FP32 Kahan terms (run after an fp32_kahan16 job)")
     ap.add_argument("--fp64-steps", type=int, default=500)
     ap.add_argument("--acc-shots", type=int, default=0, help="shots checked against FP64 (0 = all)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="host time of the CPU baseline sample")
@@ -157,6 +163,19 @@ def main():
         f"-> {achieved:.0f} GB/s; all shots active: {ms_all_active:.4f} ms, "
         f"{bytes_full / (ms_all_active * 1e-3) / 1e9:.0f} GB/s")
 
+    # the same job with the Kahan terms in FP32 (same launch shape), for comparison
+    k32 = None
+    if args.precision == "fp32_kahan16" and not args.no_kahan32:
+        rk = gpu.Run(op, md, case["src"], K, case["rec"], precision=gpu.FP32_KAHAN, nsteps=nsteps, **kw)
+        rk.advance(K)
+        ms_k = rk.last_ms_per_step
+        k32_u, k32_w = rk.seismograms()
+        bytes_k = float(np.mean([rk.step_bytes(a) for a in active]))
+        rk.close()
+        k32 = {"precision": "fp32_kahan", "value": dof_steps / (ms_k * K * 1e-3), "ms_per_step": ms_k,
+               "roofline_achieved_GBs": bytes_k / (ms_k * 1e-3) / 1e9, "bytes_per_launch": bytes_k}
+        log(f"batched fp32_kahan (FP32 Kahan terms): {ms_k:.3f} ms/step, {k32['value'] / 1e9:.2f} G DOF-steps/s")
+
     # plain single-shot FP64 loop on the same mesh (the reference-exact path)
     r64 = gpu.Run(op, md, case["src"][:1], args.fp64_steps, case["rec"], precision=gpu.FP64, **kw64)
     r64.advance(args.fp64_steps)
@@ -174,14 +193,24 @@ def main():
         ra.advance(K)
         u64, w64 = ra.seismograms()
         ra.close()
-        errs = []
-        for q, s in enumerate(idx):
-            k = int(nsteps[s])
-            a = np.concatenate([seis_u[s, :k].ravel(), seis_w[s, :k].ravel()])
-            b = np.concatenate([u64[q, :k].ravel(), w64[q, :k].ravel()])
-            errs.append(rel_l2(a, b))
+        def shot_errors(su, sw):
+            errs = []
+            for q, s in enumerate(idx):
+                k = int(nsteps[s])
+                a = np.concatenate([su[s, :k].ravel(), sw[s, :k].ravel()])
+                b = np.concatenate([u64[q, :k].ravel(), w64[q, :k].ravel()])
+                errs.append(rel_l2(a, b))
+            return errs
+
+        errs = shot_errors(seis_u, seis_w)
         acc = max(errs)
         log(f"accuracy: max rel-L2 vs FP64 over {len(idx)} shots: {acc:.3e} (median {np.median(errs):.3e})")
+        if k32 is not None:
+            e32 = shot_errors(k32_u, k32_w)
+            k32["accuracy_max_rel_l2"] = max(e32)
+            log(f"accuracy fp32_kahan: max rel-L2 {max(e32):.3e} (median {np.median(e32):.3e})")
+    if k32 is not None:
+        del k32_u, k32_w
 
     # config 4 part 2 in the same launch as part 1: 16 Q4 models of uneven size
     multi = None
@@ -283,7 +312,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "fp32_kahan": "f32"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp32_kahan": "f32", "fp32_kahan16": "f32"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"{args.shots} shots x Q4 {args.nelem}x{args.nelem} (dx=10 m), "
                                f"uneven steps U[{int(nsteps.min())},{K}], {args.precision}",
@@ -301,6 +330,7 @@ def main():
                      "all_shots_active": {"ms_per_launch": ms_all_active, "bytes_per_launch": bytes_full,
                                           "achieved": bytes_full / (ms_all_active * 1e-3) / 1e9}},
         "fp64_single": {"value": fp64_rate, "ms_per_step": ms64, "speedup_batched_vs_fp64": value / fp64_rate},
+        "fp32_kahan_terms": None if k32 is None else dict(k32, speedup_vs_fp64=k32["value"] / fp64_rate),
         "accuracy_max_rel_l2": acc,
         "multi_model": multi,
         "cpu_baseline": cpu,

@@ -34,8 +34,9 @@ enum {
 
 /* Arithmetic of a run.  FP64: bit-identical to the reference's FP64 loop.
  * FP32: single precision state and arithmetic.  FP32_KAHAN: FP32 with a
- * compensated displacement update. */
-enum { SFG_FP64 = 0, SFG_FP32 = 1, SFG_FP32_KAHAN = 2 };
+ * compensated displacement update.  FP32_KAHAN16: the same update with its
+ * compensation terms kept as bfloat16 (half the bytes of the terms). */
+enum { SFG_FP64 = 0, SFG_FP32 = 1, SFG_FP32_KAHAN = 2, SFG_FP32_KAHAN16 = 3 };
 
 /* Last error message of the calling thread ("" if none). */
 const char *sfg_last_error(void);
@@ -116,7 +117,7 @@ void sfg_medium_destroy(sfg_medium *md);
 int sfg_medium_set_row_scale(sfg_medium *md, const double *row_scale);
 
 typedef struct {
-    int precision;          /* SFG_FP64 / SFG_FP32 / SFG_FP32_KAHAN */
+    int precision;          /* SFG_FP64 / SFG_FP32 / SFG_FP32_KAHAN / SFG_FP32_KAHAN16 */
     int nshots;             /* shots sharing the operator (right-hand sides) */
     const int *src_node;    /* [nshots] source node per shot (elastic_wave.c:448) */
     const int *nsteps;      /* [nshots] executed steps per shot, or NULL = max_steps for all */

@@ -22,7 +22,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def run_case(name, case, forms=("fp64", "fp32", "fp32_kahan"), sources=None, dt_safety=0.8):
+def run_case(name, case, forms=("fp64", "fp32", "fp32_kahan", "fp32_kahan16"), sources=None, dt_safety=0.8):
     """The configured dt is checked against dt_max = 2/sqrt(lambda_max(M_L^-1 K))
     (sfg_estimate_dt); a dt above 0.95 dt_max is replaced by dt_safety * dt_max
     (rounded down to 1 us) with the step count scaled to the same simulated

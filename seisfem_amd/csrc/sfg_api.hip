@@ -375,6 +375,9 @@ extern "C" void sfg_medium_destroy(sfg_medium *md)
 // ---------------------------------------------------------------------------
 // runs
 
+// 4- or 8-byte words per (node, shot) of one Kahan-term buffer
+static size_t comp_fields(int prec) { return prec == 3 ? 3 : prec == 2 ? 6 : 0; }
+
 struct sfg_run {
     const sfg_operator *op = nullptr;
     const sfg_medium *md = nullptr;
@@ -502,7 +505,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     g_err.clear();
     if (!out || !op || !media || nmedia <= 0 || !d) return fail(SFG_E_ARG, "run: bad arguments");
     *out = nullptr;
-    if (d->precision < 0 || d->precision > 2) return fail(SFG_E_ARG, "run: unknown precision %d", d->precision);
+    if (d->precision < 0 || d->precision > 3) return fail(SFG_E_ARG, "run: unknown precision %d", d->precision);
     if (d->kernel_variant < 0 || d->kernel_variant > 7)
         return fail(SFG_E_ARG, "run: unknown kernel_variant %d", d->kernel_variant);
     if (d->xcd_remap < -1 || d->xcd_remap > (1 << 20))
@@ -712,7 +715,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     int rc = SFG_OK;
     hipError_t e = hipSuccess;
     if ((rc = dalloc((char **)&r->state, 2 * NFIELD * F * w)) ||
-        (r->prec == 2 && (rc = dalloc((char **)&r->comp, 2 * 6 * F * w))) ||
+        (r->prec >= 2 && (rc = dalloc((char **)&r->comp, 2 * comp_fields(r->prec) * F * w))) ||
         (r->prec != 0 && ((rc = dalloc((char **)&r->uw, 2 * 2 * F * w)) || (rc = dalloc(&r->pml, n)))) ||
         (rc = dalloc(&r->ctrl, 1)) ||
         (rc = dalloc(&r->active, d->max_steps)) || (rc = dalloc(&r->src, n + 1)) ||
@@ -730,7 +733,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (e == hipSuccess) e = hipEventCreate(&r->ev0);
     if (e == hipSuccess) e = hipEventCreate(&r->ev1);
     if (e == hipSuccess) e = launch_fill_zero(r->state, 2 * NFIELD * F * w, r->stream);
-    if (e == hipSuccess && r->comp) e = launch_fill_zero(r->comp, 2 * 6 * F * w, r->stream);
+    if (e == hipSuccess && r->comp) e = launch_fill_zero(r->comp, 2 * comp_fields(r->prec) * F * w, r->stream);
     if (e == hipSuccess && r->uw) e = launch_fill_zero(r->uw, 2 * 2 * F * w, r->stream);
     if (e == hipSuccess) e = launch_fill_zero(r->ctrl, sizeof(Ctrl), r->stream);
     if (e == hipSuccess) e = launch_fill_zero(r->seis_u, seis_n * 8, r->stream);
@@ -950,7 +953,8 @@ extern "C" double sfg_run_step_bytes(const sfg_run *r, int active)
 {
     if (!r) return 0.0;
     const double N = r->op->n, nnz = r->op->nnz, w = r->prec == 0 ? 8.0 : 4.0;
-    const double K = r->prec == 2 ? 2.0 * 6.0 * w : 0.0;
+    // Kahan terms read and written: 6 words, or 6 bfloat16 (3 words) in precision 3
+    const double K = r->prec >= 2 ? 2.0 * comp_fields(r->prec) * w : 0.0;
     // per-shot media read 12 coefficients per (node, shot) instead of 11 per node
     const double coef = r->cps ? (double)active * N * 12.0 * w : 11.0 * N * w;
     return 4.0 * (N + 1) + nnz * (4.0 + 7.0 * w) + coef + (double)active * N * (52.0 * w + K);
@@ -1068,7 +1072,7 @@ extern "C" int sfg_estimate_dt(const sfg_operator *op, const sfg_medium *md, int
 struct sfg_multi {
     std::vector<sfg_run *> runs;
     int prec = 0, block = 256, ntiles = 0, max_steps = 0, guard_every = 50, executed = 0;
-    int xcd_remap = 1;          // the first run's flag: contiguous tile ranges per XCD, or round-robin
+    int xcd_remap = 1;          // the first run's tile map: 0 round-robin, 1 contiguous, Q + 1 stripes of Q
     Ctrl *ctrl = nullptr;       // shared step counter
     void *args = nullptr;       // device StepArgs<T>[nruns]
     int *tile_start = nullptr;  // device [nruns + 1]
@@ -1129,7 +1133,11 @@ extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns
     m->block = r0->block;
     m->max_steps = r0->max_steps;
     m->guard_every = r0->guard_every;
-    m->xcd_remap = r0->prec == 0 ? r0->a64.xcd_remap : r0->a32.xcd_remap;
+    {
+        const int remap = r0->prec == 0 ? r0->a64.xcd_remap : r0->a32.xcd_remap;
+        const int stripe = r0->prec == 0 ? r0->a64.xcd_stripe : r0->a32.xcd_stripe;
+        m->xcd_remap = stripe > 0 ? stripe + 1 : remap;
+    }
     std::vector<int> ts(nruns + 1, 0);
     for (int k = 0; k < nruns; ++k) {
         m->runs.push_back(runs[k]);

@@ -123,7 +123,8 @@ struct StepArgs {
     Ctrl *ctrl;
 };
 
-// Launchers (defined in sfg_kernels.hip).  precision: 0 exact FP64, 1 FP32, 2 FP32+Kahan.
+// Launchers (defined in sfg_kernels.hip).  precision: 0 exact FP64, 1 FP32, 2 FP32+Kahan,
+// 3 FP32+Kahan with the compensation terms stored as bfloat16.
 // slot: parity of the step being launched (see Ctrl).
 hipError_t launch_step(int precision, int sp, bool coef_per_shot, const void *args, int nblk_x,
                        int nblk_y, int block, int slot, size_t lds_bytes, hipStream_t st);

@@ -50,6 +50,18 @@ __device__ __forceinline__ void st_once(T *p, T v)
     else *p = v;
 }
 
+// bfloat16 halves of a 32-bit word (precision 3 keeps the Kahan terms of two
+// displacement fields per word): the low half is field 2k, the high half 2k+1.
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+// round to nearest even (the terms are finite: a Kahan term is the rounding
+// error of an FP32 sum)
+__device__ __forceinline__ uint32_t bf16_bits(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
 // Read-only arrays of a step (pattern, values, coefficients, schedules) are
 // read through the constant address space: they do not change during a
 // launch, which lets wave-uniform reads (one row per wave) go through the
@@ -126,7 +138,8 @@ template <class T, int PREC, int SP, bool CPS, int V>
 __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, int by)
 {
     constexpr bool EXACT = (PREC == 0);
-    constexpr bool KAHAN = (PREC == 2);
+    constexpr bool KAHAN = (PREC >= 2);
+    constexpr bool C16 = (PREC == 3);           // Kahan terms stored as bfloat16 pairs
     constexpr bool LDSG = V == 6;
     constexpr bool PACKED = V >= 1;
     constexpr bool NT = V >= 2;
@@ -212,12 +225,16 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         if constexpr (!EXACT) pk = ro(a.pml)[row];
         const bool interior = SKIP && pk < 2;    // short gather
         const bool keepv = EXACT || pk != 0;     // velocity stored
-        // Kahan terms: 6 fields in the same layout as the state
-        const size_t C6 = (size_t)6 * F;
+        // Kahan terms: 6 fields in the same layout as the state, or (C16) 3
+        // words of two bfloat16 terms each
+        constexpr int NC = C16 ? 3 : 6;
+        const size_t C6 = (size_t)NC * F;
         const T *cin = KAHAN ? a.comp + (size_t)(t & 1) * C6 : nullptr;
         T *cout = KAHAN ? a.comp + (size_t)((t + 1) & 1) * C6 : nullptr;
+        const uint32_t *win = reinterpret_cast<const uint32_t *>(cin);
+        uint32_t *wout = reinterpret_cast<uint32_t *>(cout);
         size_t co;
-        if constexpr (NM) co = ((size_t)row * (S / GS) + by) * (6 * GS) + ls;
+        if constexpr (NM) co = ((size_t)row * (S / GS) + by) * (NC * GS) + ls;
         else co = node_base(row);
         // own-row data nobody else gathers: with CAP it is requested before the
         // gathers so that its latency overlaps theirs
@@ -226,7 +243,14 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             const size_t o0 = node_base(row);
 #pragma unroll
             for (int q = 0; q < 6; ++q) pre_acc[q] = ld_once<NT>(in + (U1TT + q) * FS + o0);
-            if constexpr (KAHAN) {
+            if constexpr (C16) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const uint32_t w = ld_once<NT>(win + k * FS + co);
+                    pre_c[2 * k] = bf16_lo(w);
+                    pre_c[2 * k + 1] = bf16_hi(w);
+                }
+            } else if constexpr (KAHAN) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) pre_c[q] = ld_once<NT>(cin + q * FS + co);
             }
@@ -465,7 +489,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
 
         // ---- Newmark update (:757-776) ---------------------------------------
         const T ca = a.ca, al = a.alpha, cd = a.cd, de = a.delta;
-        T dn[6];
+        T dn[6], cn[6];
+        if constexpr (C16 && !PRE) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t w = ld_once<NT>(win + k * FS + co);
+                pre_c[2 * k] = bf16_lo(w);
+                pre_c[2 * k + 1] = bf16_hi(w);
+            }
+        }
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             const T d = cur[U1 + q];
@@ -479,9 +511,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                 const T p = cur[U1TT + q];
                 if constexpr (KAHAN) {
                     const T inc = p * dt + al * an[q] * dt * dt;
-                    const T y = inc - (PRE ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
+                    const T y = inc - ((PRE || C16) ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
                     const T sum = d + y;
-                    st_once<NT>(cout + q * FS + co, (sum - d) - y);
+                    cn[q] = (sum - d) - y;
+                    if constexpr (!C16) st_once<NT>(cout + q * FS + co, cn[q]);
                     dn[q] = sum;
                 } else {
                     dn[q] = d + p * dt + al * an[q] * dt * dt;
@@ -490,6 +523,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                 st_once<NT>(out + (U1TT + q) * FS + o, p + (de + ca) * an[q] * dt);
             }
             st_once<NT>(out + (U1 + q) * FS + o, dn[q]);
+        }
+        if constexpr (C16) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                st_once<NT>(wout + k * FS + co, bf16_bits(cn[2 * k]) | (bf16_bits(cn[2 * k + 1]) << 16));
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) st_once<NT>(out + (LX1 + q) * FS + o, ln[q]);
@@ -551,7 +589,9 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
     const int t = begin_step(ctrl, slot);
     const unsigned int nb = gridDim.x;
     unsigned int bid = blockIdx.x;
-    if (xcd_remap) bid = xcd_contiguous(bid, nb);
+    // xcd_remap: 0 round-robin, 1 contiguous ranges, Q + 1 stripes of Q tiles
+    if (xcd_remap > 1) bid = xcd_striped(bid, nb, (unsigned int)(xcd_remap - 1));
+    else if (xcd_remap) bid = xcd_contiguous(bid, nb);
     int r = 0;
     while (r + 1 < nruns && (int)bid >= tile_start[r + 1]) ++r;
     const StepArgs<T> &a = args[r];
@@ -589,6 +629,9 @@ hipError_t launch_multi_step(int precision, const void *args_dev, const int *til
                            (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
     else if (precision == 2)
         hipLaunchKernelGGL((multi_step_kernel<float, 2>), dim3(ntiles), dim3(block), 0, st,
+                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
+    else if (precision == 3)
+        hipLaunchKernelGGL((multi_step_kernel<float, 3>), dim3(ntiles), dim3(block), 0, st,
                            (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
     else
         return hipErrorInvalidValue;
@@ -666,6 +709,7 @@ hipError_t allow_step_lds(int precision, int sp, bool cps, size_t lds_bytes)
     case 0: return allow_lds_t<double, 0>(sp, cps, (int)lds_bytes);
     case 1: return allow_lds_t<float, 1>(sp, cps, (int)lds_bytes);
     case 2: return allow_lds_t<float, 2>(sp, cps, (int)lds_bytes);
+    case 3: return allow_lds_t<float, 3>(sp, cps, (int)lds_bytes);
     default: return hipErrorInvalidValue;
     }
 }
@@ -678,6 +722,7 @@ hipError_t launch_step(int precision, int sp, bool cps, const void *args, int nb
     case 0: return launch_step_t<double, 0>(sp, cps, args, g, block, slot, lds_bytes, st);
     case 1: return launch_step_t<float, 1>(sp, cps, args, g, block, slot, lds_bytes, st);
     case 2: return launch_step_t<float, 2>(sp, cps, args, g, block, slot, lds_bytes, st);
+    case 3: return launch_step_t<float, 3>(sp, cps, args, g, block, slot, lds_bytes, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -11,8 +11,8 @@ import numpy as np
 
 from ._native import NativeError, ShotDesc, check, lib  # noqa: F401  (NativeError re-exported)
 
-FP64, FP32, FP32_KAHAN = 0, 1, 2
-PRECISION = {"fp64": FP64, "fp32": FP32, "fp32_kahan": FP32_KAHAN}
+FP64, FP32, FP32_KAHAN, FP32_KAHAN16 = 0, 1, 2, 3
+PRECISION = {"fp64": FP64, "fp32": FP32, "fp32_kahan": FP32_KAHAN, "fp32_kahan16": FP32_KAHAN16}
 
 
 def _pi(a):

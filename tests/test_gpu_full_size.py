@@ -67,10 +67,11 @@ def test_full_size_mirror_symmetry(cfg1):
     assert np.max(np.abs(w + w[:, ::-1])) < 1e-9 * su
 
 
-def test_full_size_fp32_kahan_within_1e4_of_fp64(cfg1):
-    say("config 1 full size, FP32+Kahan vs FP64")
+@pytest.mark.parametrize("prec", [gpu.FP32_KAHAN, gpu.FP32_KAHAN16])
+def test_full_size_fp32_kahan_within_1e4_of_fp64(cfg1, prec):
+    say(f"config 1 full size, FP32+Kahan (precision {prec}) vs FP64")
     u64, w64 = _run(cfg1, gpu.FP64)
-    u32, w32 = _run(cfg1, gpu.FP32_KAHAN)
+    u32, w32 = _run(cfg1, prec)
     a = np.concatenate([u32.ravel(), w32.ravel()])
     b = np.concatenate([u64.ravel(), w64.ravel()])
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-4

@@ -202,7 +202,7 @@ def test_kernel_variants_agree_bit_for_bit(sp, prec, block):
     op.close()
 
 
-@pytest.mark.parametrize("prec", [gpu.FP32, gpu.FP32_KAHAN])
+@pytest.mark.parametrize("prec", [gpu.FP32, gpu.FP32_KAHAN, gpu.FP32_KAHAN16])
 def test_fp32_interior_rows_short_gather_equals_full_walk(prec):
     """FP32 forms: a row outside every damping layer gathers U, W and the
     memory fields only (10 values per neighbour instead of 20).  The terms it
@@ -368,7 +368,7 @@ def test_distributed_sources():
     op.close()
 
 
-@pytest.mark.parametrize("prec,tol", [(gpu.FP32, 1e-4), (gpu.FP32_KAHAN, 1e-4)])
+@pytest.mark.parametrize("prec,tol", [(gpu.FP32, 1e-4), (gpu.FP32_KAHAN, 1e-4), (gpu.FP32_KAHAN16, 1e-4)])
 def test_fp32_within_tolerance_of_fp64_oracle(prec, tol):
     """FP32 seismograms within rel-L2 1e-4 of the FP64 oracle (the stated bar)."""
     case = _case(host.Q4, 48, 900)
