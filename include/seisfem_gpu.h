@@ -147,8 +147,9 @@ typedef struct {
                                   accelerations / Kahan terms loaded first, 5 and 6 = 4 with 3 and 4
                                   nonzeros' gathers in flight per trip, 7 = LDS-staged gathers: each
                                   block copies the fields of its rows' neighbour nodes into LDS once
-                                  (16, 32 or 64 shots per lane group only).  Same results bit for
-                                  bit; a multi-run launch runs 3 to 7 as 1 */
+                                  (16, 32 or 64 shots per lane group only).  Same results (bit for
+                                  bit in FP64; in the FP32 forms variant 7 can differ in the sign
+                                  of a zero); a multi-run launch runs 4 to 6 as 4, 3 and 7 as 1 */
     const int *row_order;      /* [node_num] permutation: the order in which the step kernel's
                                   waves take rows (e.g. sfh_locality_order), NULL = node order.
                                   Changes only which rows run together, not any result */
