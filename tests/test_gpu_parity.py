@@ -288,6 +288,50 @@ def test_per_shot_media_match_their_own_oracle_runs():
     op.close()
 
 
+def test_per_shot_media_fp32_forms_within_tolerance():
+    """Per-shot media in the FP32 form with bfloat16 Kahan terms (per-shot
+    M-PML profiles decide the short gather and the rows that keep a velocity):
+    every shot within rel-L2 1e-4 of the FP64 run of the same media, and the
+    same numbers from 4-shot field-major and 64-shot node-major lane groups."""
+    nelem, steps = 14, 120
+    models = [(2000.0, 3000.0), (2300.0, 3400.0), (1800.0, 2600.0)]
+    cases = []
+    for rho0, vp0 in models:
+        def model(depth, y, rho0=rho0, vp0=vp0):
+            vp = vp0 + 0.1 * depth
+            return workloads.isotropic_model(rho0 + 0.05 * depth, vp, vp / 1.73)
+        cases.append(workloads.structured_case(host.Q4, nelem, nelem, 10.0, model, 3, 5e-4, 25.0, 0.04, steps,
+                                               20.0, [50.0, 70.0, 90.0], 9, 0.0, 140.0))
+    say("per-shot media: 3 models on Q4 14x14, FP32 + bfloat16 Kahan terms vs FP64")
+    p0 = cases[0]["problem"]
+    op = gpu.Operator(p0["node_num"], p0["csr"])
+    mds = []
+    for cs in cases:
+        p = cs["problem"]
+        md = gpu.Medium(op, p["c"], p["mass_lump"], p["dirichlet"], p["mpml"])
+        md.set_row_scale(p["rho"] / p0["rho"])
+        mds.append(md)
+    src = np.array([cs["src"][k] for k, cs in enumerate(cases)], np.int32)
+    rec, dt, f0, t0 = cases[0]["rec"], cases[0]["dt"], cases[0]["f0"], cases[0]["t0"]
+    res = []
+    for prec, kw in [(gpu.FP64, {}), (gpu.FP32_KAHAN16, dict(shots_per_lane_group=4)),
+                     (gpu.FP32_KAHAN16, dict(shots_per_lane_group=64, kernel_variant=5, block=64))]:
+        run = gpu.Run(op, mds, src, steps, rec, dt, f0, t0, precision=prec, **kw)
+        run.advance(steps)
+        res.append(run.seismograms())
+        run.close()
+    (u64, w64), (ua, wa), (ub, wb) = res
+    assert np.array_equal(ua, ub) and np.array_equal(wa, wb)
+    for k in range(len(cases)):
+        a = np.concatenate([ua[k].ravel(), wa[k].ravel()])
+        b = np.concatenate([u64[k].ravel(), w64[k].ravel()])
+        assert np.max(np.abs(b)) > 0
+        assert rel_l2(a, b) <= 1e-4, (k, rel_l2(a, b))
+    for md in mds:
+        md.close()
+    op.close()
+
+
 def test_multi_run_one_launch_per_step_matches_separate_runs():
     """Four runs on different meshes (single-shot, 70 shots in 64-shot groups,
     45 shots in 32-shot groups with the unroll-3 variant; uneven step counts)
