@@ -144,8 +144,9 @@ typedef struct {
     int kernel_variant;        /* 0 = default (1), 1 = packed values + 2-way gather, 2 = separate value
                                   arrays, 3 = 1 + streaming hints on data used once per step,
                                   4 = 3 + own-row fields kept from the diagonal gather and own-row
-                                  accelerations / Kahan terms loaded first, 5 and 6 = 4 with 3 and 4
-                                  nonzeros' gathers in flight per trip, 7 = LDS-staged gathers: each
+                                  accelerations (p in the FP32 forms) / Kahan terms loaded first,
+                                  5 and 6 = 4 with 3 and 4 nonzeros' gathers in flight per trip,
+                                  7 = LDS-staged gathers: each
                                   block copies the fields of its rows' neighbour nodes into LDS once
                                   (16, 32 or 64 shots per lane group only).  Same results (bit for
                                   bit in FP64; in the FP32 forms variant 7 can differ in the sign
