@@ -599,9 +599,11 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
     const int bx = local % a.nbx, by = local / a.nbx;
     // Kernel variants of one launch: separate value arrays (0), packed values
     // (1; also runs 2 and 6), and for 32- and 64-shot groups the form with
-    // own-row data first (3; also runs 4 and 5).  The launch's register count is
-    // that of its largest form, so the deeper unrolls are left out: they would
-    // cost every run a wave per SIMD.
+    // own-row data first (runs 3, 4 and 5): 3 nonzeros per trip in the FP32
+    // forms, 2 in FP64.  The launch's register count is that of its largest
+    // form; deeper unrolls would cost every run a wave per SIMD (FP64 with 3
+    // per trip: 256 VGPRs, one wave).
+    constexpr int D = PREC == 0 ? 3 : 4;
     const int v = a.variant;
     const int w = v == 0 ? 0 : (v >= 3 && v <= 5) ? 3 : 1;
 #define SFG_T(SPV, C, VV) step_tile<T, PREC, SPV, C, VV>(a, t, bx, by)
@@ -609,11 +611,11 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
         if (a.cps) { if (w) SFG_T(1, true, 1); else SFG_T(1, true, 0); }
         else { if (w) SFG_T(1, false, 1); else SFG_T(1, false, 0); }
     } else if (a.sp == 32) {
-        if (a.cps) { if (w == 3) SFG_T(32, true, 3); else if (w) SFG_T(32, true, 1); else SFG_T(32, true, 0); }
-        else { if (w == 3) SFG_T(32, false, 3); else if (w) SFG_T(32, false, 1); else SFG_T(32, false, 0); }
+        if (a.cps) { if (w == 3) SFG_T(32, true, D); else if (w) SFG_T(32, true, 1); else SFG_T(32, true, 0); }
+        else { if (w == 3) SFG_T(32, false, D); else if (w) SFG_T(32, false, 1); else SFG_T(32, false, 0); }
     } else {
-        if (a.cps) { if (w == 3) SFG_T(64, true, 3); else if (w) SFG_T(64, true, 1); else SFG_T(64, true, 0); }
-        else { if (w == 3) SFG_T(64, false, 3); else if (w) SFG_T(64, false, 1); else SFG_T(64, false, 0); }
+        if (a.cps) { if (w == 3) SFG_T(64, true, D); else if (w) SFG_T(64, true, 1); else SFG_T(64, true, 0); }
+        else { if (w == 3) SFG_T(64, false, D); else if (w) SFG_T(64, false, 1); else SFG_T(64, false, 0); }
     }
 #undef SFG_T
 }
