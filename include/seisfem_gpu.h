@@ -145,7 +145,8 @@ typedef struct {
                                   arrays, 3 = 1 + streaming hints on data used once per step,
                                   4 = 3 + own-row fields kept from the diagonal gather and own-row
                                   accelerations (p in the FP32 forms) / Kahan terms loaded first,
-                                  5 and 6 = 4 with 3 and 4 nonzeros' gathers in flight per trip,
+                                  5 and 6 = 4 with 3 and 4 nonzeros' gathers in flight per trip
+                                  (6: 2 per trip on rows with an M-PML profile in the FP32 forms),
                                   7 = LDS-staged gathers: each
                                   block copies the fields of its rows' neighbour nodes into LDS once
                                   (16, 32 or 64 shots per lane group only).  Same results (bit for

@@ -146,6 +146,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
     constexpr bool PRE = V >= 3;                 // own-row accelerations / Kahan terms first
     constexpr bool CAP = V >= 3 && !LDSG;        // own-row fields from the diagonal gather
     constexpr int UN = V == 5 ? 4 : V == 4 ? 3 : 2;  // nonzeros per trip of the gather loop
+    // the full 20-field walk of M-PML rows in variant 5 keeps 2 per trip: its
+    // registers, not the short walk's, set the kernel's occupancy
+    constexpr int UNP = (V == 5 && !EXACT) ? 2 : UN;
     constexpr bool WUW = !EXACT;                 // store U and W of the new state
     constexpr bool SKIP = !EXACT && !LDSG;       // interior rows gather 10 values per neighbour
     constexpr int RPW = (SP >= 64) ? 1 : 64 / SP;
@@ -357,14 +360,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             mly1 = mac<EXACT>(mly1, am, ly1); mly2 = mac<EXACT>(mly2, am, ly2);
             mly3 = mac<EXACT>(mly3, am, ly3); mly4 = mac<EXACT>(mly4, am, ly4);
         };
-        auto walk = [&](auto &&nz) __attribute__((always_inline)) {
+        auto walk = [&](auto &&nz, auto per_trip) __attribute__((always_inline)) {
+            constexpr int U = decltype(per_trip)::value;
             if constexpr (PACKED) {
-                // UN nonzeros per trip: their gathers are in flight together; each
+                // U nonzeros per trip: their gathers are in flight together; each
                 // sum still runs in column order
                 int k = k0;
-                for (; k + UN - 1 < k1; k += UN) {
+                for (; k + U - 1 < k1; k += U) {
 #pragma unroll
-                    for (int u = 0; u < UN; ++u) nz(k + u);
+                    for (int u = 0; u < U; ++u) nz(k + u);
                 }
                 for (; k < k1; ++k) nz(k);
             } else {
@@ -377,9 +381,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
 #pragma unroll
                 for (int f = U1; f <= W3; ++f) cur[f] = ld_once<NT>(in + f * FS + o);
             }
-            walk(nonzero_in);
+            walk(nonzero_in, std::integral_constant<int, UN>{});
         } else {
-            walk(nonzero);
+            walk(nonzero, std::integral_constant<int, UNP>{});
         }
 
         // ---- per-node coefficients --------------------------------------
