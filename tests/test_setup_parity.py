@@ -3,8 +3,8 @@ reference.
 
 Pins, strongest first:
   * the reference's own sources compiled unchanged (oracle/_ref), bit for bit;
-  * the CSR sizes the reference produced for full-size meshes (measured with
-    the reference build and recorded with the survey), which pin its
+  * the CSR sizes the reference produced for full-size meshes (counted with a
+    build of the reference and written into this file), which pin its
     exact-zero elimination;
   * the coo2csr known-answer test of sparse_matrix/test_coo2csr_lib.c:10-12,
     with the expected CSR worked out by hand;
