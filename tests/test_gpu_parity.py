@@ -209,7 +209,7 @@ def test_fp32_interior_rows_short_gather_equals_full_walk(prec):
     skips enter its equations multiplied by M-PML profiles that are exactly
     zero, so seismograms at every node and the final wavefield equal those of
     the full walk (the LDS-staged variant 7, which always gathers all 20) for
-    lane groups of 1, 32 and 64 shots."""
+    lane groups of 1, 4, 16, 32 and 64 shots."""
     case = _case(host.Q4, 12, 50, nshots=40)
     prob = case["problem"]
     n = prob["node_num"]
@@ -219,7 +219,8 @@ def test_fp32_interior_rows_short_gather_equals_full_walk(prec):
     op = gpu.Operator(n, prob["csr"])
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
     res = []
-    for sp, v, block in [(64, 7, 64), (64, 5, 64), (64, 1, 256), (32, 6, 128), (1, 1, 64)]:
+    for sp, v, block in [(64, 7, 64), (64, 5, 64), (64, 1, 256), (32, 6, 128), (16, 5, 64), (4, 4, 128),
+                         (1, 1, 64)]:
         run = gpu.Run(op, md, case["src"], 50, np.arange(n, dtype=np.int32), case["dt"], case["f0"], case["t0"],
                       precision=prec, shots_per_lane_group=sp, guard_every=25, kernel_variant=v, block=block)
         run.advance(50)
