@@ -10,10 +10,16 @@ checked through properties that hold at any size:
   truncated pi);
 * the FP32 + Kahan loop stays within 1e-4 relative L2 of the FP64 loop (u and
   w concatenated), the accuracy bar of the batched forms.
+
+The full-size meshes of configurations 1-3 (Q4 200x200, Q9 400x400, T6 with
+topography) also run their first 100 FP64 steps against the oracle, bit for
+bit, at every node within 60 m of the source (a point force; the oracle has no
+distributed source).
 """
 import numpy as np
 import pytest
 
+import oracle
 from seisfem_amd import gpu, workloads
 
 pytestmark = pytest.mark.gpu
@@ -75,3 +81,26 @@ def test_full_size_fp32_kahan_within_1e4_of_fp64(cfg1, prec):
     a = np.concatenate([u32.ravel(), w32.ravel()])
     b = np.concatenate([u64.ravel(), w64.ravel()])
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-4
+
+
+def _bits(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+@pytest.mark.parametrize("name", ["config1", "config2", "config3"])
+def test_full_size_fp64_first_steps_bit_identical_to_oracle(name):
+    case = getattr(workloads, name)()
+    prob = case["problem"]
+    s = int(case["src"][0])
+    x, y = prob["x"], prob["y"]
+    rec = np.where(np.hypot(x - x[s], y - y[s]) <= 60.0)[0].astype(np.int32)
+    steps = 100
+    say(f"{name} full size ({prob['node_num']} nodes), FP64, {steps} steps, {rec.size} receivers around node {s}")
+    res = gpu.simulate(prob, np.array([s], np.int32), steps, rec, case["dt"], case["f0"], case["t0"],
+                       precision=gpu.FP64)
+    ref = oracle.oracle_masslump(prob, s, steps + 2, rec, case["dt"], case["f0"], case["t0"])
+    assert ref["status"] == steps
+    assert np.max(np.abs(ref["seis_u"])) > 0 and np.max(np.abs(ref["seis_w"])) > 0
+    assert _bits(res["seis_u"][0], ref["seis_u"]) and _bits(res["seis_w"][0], ref["seis_w"])
