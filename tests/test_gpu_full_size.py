@@ -104,3 +104,33 @@ def test_full_size_fp64_first_steps_bit_identical_to_oracle(name):
     assert ref["status"] == steps
     assert np.max(np.abs(ref["seis_u"])) > 0 and np.max(np.abs(ref["seis_w"])) > 0
     assert _bits(res["seis_u"][0], ref["seis_u"]) and _bits(res["seis_w"][0], ref["seis_w"])
+
+
+def test_batched_fp64_bench_shape_bit_identical_to_oracle():
+    """The batched workload's shape (128 shots on Q4 400x400, 64-shot lane
+    groups, kernel variant 5, XCD stripes of 12 row blocks, uneven step counts)
+    in FP64: three shots, 60 steps, every node within 60 m of each source,
+    bit-identical to the oracle's single-shot runs."""
+    case = workloads.config4a()
+    prob = case["problem"]
+    x, y = prob["x"], prob["y"]
+    steps = 60
+    nsteps = np.clip(case["nsteps"] - 1510, 20, steps).astype(np.int32)   # uneven, 20..60 steps
+    nsteps[[0, 64, 127]] = steps
+    near = lambda s: np.hypot(x - x[s], y - y[s]) <= 60.0
+    rec = np.where(near(int(case["src"][0])) | near(int(case["src"][64])) | near(int(case["src"][127])))[0]
+    rec = rec.astype(np.int32)
+    say(f"config 4a batched FP64: 128 shots, steps {int(nsteps.min())}..{steps}, {rec.size} receivers")
+    op = gpu.Operator(prob["node_num"], prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    run = gpu.Run(op, md, case["src"], steps, rec, case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
+                  nsteps=nsteps, shots_per_lane_group=64, block=256, kernel_variant=5, xcd_remap=12)
+    run.advance(steps)
+    u, w = run.seismograms()
+    run.close()
+    md.close()
+    op.close()
+    for k in (0, 64, 127):
+        ref = oracle.oracle_masslump(prob, int(case["src"][k]), steps + 2, rec, case["dt"], case["f0"], case["t0"])
+        assert np.max(np.abs(ref["seis_u"])) > 0
+        assert _bits(u[k], ref["seis_u"]) and _bits(w[k], ref["seis_w"]), k
