@@ -107,6 +107,22 @@ def test_example6_on_gpu_character_identical():
         os.path.join(out, "wavefield_w_shot_1.txt"))
 
 
+def test_example3_on_gpu_character_identical():
+    """The reference's example 3 (T6, second-order triangles) through the GPU
+    FP64 loop: its shipped seismogram files, character for character."""
+    from test_oracle import _ex_problem
+    par, prob, src, rec = _ex_problem(os.path.join(GOLD, "ex3"))
+    prob = dict(prob)
+    xmin, xmax, ymin, ymax = prob["bbox"]
+    prob["mpml"] = host.abc_mpml(prob["element_node"], host.element_order(par["type_code"]), prob["x"], prob["y"],
+                                 20, 20, par["edge_size"], xmin, xmax, ymin, ymax, prob["vp_max"], prob["use_mpml"])
+    say(f"example 3 T6 {prob['node_num']} nodes, {par['step'] - 2} steps, FP64")
+    res = gpu.simulate(prob, src, par["step"] - 2, rec, par["dt"], par["f0"], par["t0"], precision=gpu.FP64)
+    out = os.path.join(GOLD, "ex3", "outputfile")
+    assert formats.format_rows(res["seis_u"][0]) == formats.read_text(os.path.join(out, "seismogram_u_shot_1.txt"))
+    assert formats.format_rows(res["seis_w"][0]) == formats.read_text(os.path.join(out, "seismogram_w_shot_1.txt"))
+
+
 @pytest.mark.parametrize("nshots,sp", [(3, 4), (11, 16), (70, 64), (37, 16), (45, 32), (5, 1)])
 def test_multishot_fp64_each_shot_bit_identical(nshots, sp):
     case = _case(host.Q4, 14, 90, nshots=nshots)
