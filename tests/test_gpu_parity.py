@@ -444,16 +444,17 @@ def test_fp32_within_tolerance_of_fp64_oracle(prec, tol):
     assert err < tol
 
 
-def test_energy_guard_reports_instead_of_exiting():
+@pytest.mark.parametrize("prec,sp", [(gpu.FP64, 1), (gpu.FP32_KAHAN16, 64)])
+def test_energy_guard_reports_instead_of_exiting(prec, sp):
     """dt far above the stability limit: the run stops with SFG_E_ENERGY
     (the reference calls exit(1)); kernels finish normally on inf/NaN."""
     case = _case(host.Q4, 12, 400)
     prob = case["problem"]
-    say("energy guard Q4 12x12 dt x8")
+    say(f"energy guard Q4 12x12 dt x8 precision={prec} sp={sp}")
     op = gpu.Operator(prob["node_num"], prob["csr"])
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
     run = gpu.Run(op, md, case["src"], 400, case["rec"], case["dt"] * 8, case["f0"], case["t0"],
-                  precision=gpu.FP64, guard_every=20)
+                  precision=prec, guard_every=20, shots_per_lane_group=sp)
     with pytest.raises(__import__("seisfem_amd").NativeError) as ei:
         run.advance(400)
     assert ei.value.code == -6
