@@ -18,6 +18,8 @@
 #ifndef SEISFEM_GPU_H
 #define SEISFEM_GPU_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif

@@ -60,3 +60,28 @@ def test_bench_refuses_more_than_one_gpu_without_touching_it():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_headers_compile_as_c99_and_link(tmp_path):
+    """A C caller (the reference's seisfem.c is C) includes the three headers
+    with -std=c99 -Wall -Werror, links the library and calls a host-only entry
+    point (no device is touched)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    assert cc, "no C compiler"
+    src = tmp_path / "caller.c"
+    src.write_text(
+        '#include "seisfem_gpu.h"\n#include "seisfem_host.h"\n#include "seisfem_surrogate.h"\n'
+        "#include <stdio.h>\n"
+        "int main(void) {\n"
+        "    sfg_shot_desc d = {0};\n"
+        "    d.precision = SFG_FP32_KAHAN16;\n"
+        '    printf("%.6f %d\\n", seismic_source(10.0, 0.1, 1.0, 0.1), d.precision);\n'
+        "    return 0;\n}\n")
+    exe = tmp_path / "caller"
+    libdir = os.path.join(ROOT, "seisfem_amd")
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe), "-L", libdir, "-lseisfem_amd", "-Wl,-rpath," + libdir], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert float(out[0]) == 1.0 and int(out[1]) == 3
