@@ -85,3 +85,25 @@ def test_headers_compile_as_c99_and_link(tmp_path):
                     str(exe), "-L", libdir, "-lseisfem_amd", "-Wl,-rpath," + libdir], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
     assert float(out[0]) == 1.0 and int(out[1]) == 3
+
+
+def test_shot_descriptor_binding_matches_the_c_layout(tmp_path):
+    """The ctypes mirror of sfg_shot_desc has the C struct's size and every
+    field at the C offset (a drift would hand the library wrong launch fields)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    assert cc, "no C compiler"
+    names = [f[0] for f in _native.ShotDesc._fields_]
+    body = "".join(f'    printf("{n} %zu\\n", offsetof(sfg_shot_desc, {n}));\n' for n in names)
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "seisfem_gpu.h"\nint main(void) {\n'
+                   '    printf("sizeof %zu\\n", sizeof(sfg_shot_desc));\n' + body + "    return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                        text=True).stdout.splitlines())
+    assert int(got["sizeof"]) == ctypes.sizeof(_native.ShotDesc)
+    for n in names:
+        assert int(got[n]) == getattr(_native.ShotDesc, n).offset, n
