@@ -386,6 +386,40 @@ def test_multi_run_one_launch_per_step_matches_separate_runs():
         r.close()
 
 
+def test_multi_run_fp32_forms_match_separate_runs():
+    """The same four runs in the FP32 form with bfloat16 Kahan terms: one
+    launch per step for all of them gives the seismograms each run gives
+    stepped on its own (a lane's arithmetic does not depend on the launch)."""
+    specs = [(10, 1, 90, 0, 0), (13, 1, 55, 0, 0), (12, 70, 90, 64, 1), (11, 45, 90, 32, 5)]
+    say("multi-run FP32 + bfloat16 Kahan terms vs separate runs")
+    grouped, alone, keep = [], [], []
+    for nelem, nshots, steps, sp, variant in specs:
+        case = _case(host.Q4, nelem, 90, nshots=nshots)
+        prob = case["problem"]
+        op = gpu.Operator(prob["node_num"], prob["csr"])
+        md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+        nsteps = np.full(nshots, steps, np.int32)
+        if nshots > 1:
+            nsteps = np.random.default_rng(1).integers(30, 91, nshots).astype(np.int32)
+            nsteps[0] = 90
+        for lst in (grouped, alone):
+            lst.append(gpu.Run(op, md, case["src"], 90, case["rec"], case["dt"], case["f0"], case["t0"],
+                               precision=gpu.FP32_KAHAN16, nsteps=nsteps, guard_every=25, block=256,
+                               shots_per_lane_group=sp, kernel_variant=variant))
+        keep += [op, md]
+    multi = gpu.Multi(grouped)
+    multi.advance(90)
+    for a, b in zip(grouped, alone):
+        b.advance(90)
+        ua, wa = a.seismograms()
+        ub, wb = b.seismograms()
+        assert np.max(np.abs(ub)) > 0
+        assert np.array_equal(ua, ub) and np.array_equal(wa, wb)
+    multi.close()
+    for r in grouped + alone:
+        r.close()
+
+
 def test_distributed_sources():
     """(1) a point force given as a one-entry distributed source is bit-identical
     to the default path; (2) an explosive source on the symmetry line of a
