@@ -472,7 +472,7 @@ static StateLayout state_layout(const sfg_run *r)
 // ctrl / stream default to the run's own (a multi-run passes its shared ones)
 static int launch_guard_now(sfg_run *r, const Ctrl *ctrl = nullptr, hipStream_t st = nullptr)
 {
-    HIPCHK(launch_guard(r->prec, r->state, r->op->n, state_layout(r), ctrl ? ctrl : r->ctrl, r->energy,
+    HIPCHK(launch_guard(r->prec, r->state, r->uw, r->op->n, state_layout(r), ctrl ? ctrl : r->ctrl, r->energy,
                         r->nslots, r->guard_every, st ? st : r->stream));
     return SFG_OK;
 }
@@ -827,6 +827,8 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.comp = comp;
         a.uw = (decltype(comp))r->uw;
         a.pml = r->pml;
+        // the LDS-staged walk reads every neighbour's split fields, so it keeps them on all rows
+        a.agg = (r->prec != 0 && d->kernel_variant != 7) ? 1 : 0;
         a.fstride = F;
         a.active = r->active;
         a.src_ptr = r->src;
@@ -990,7 +992,8 @@ extern "C" int sfg_run_snapshot(sfg_run *r, int shot, double *U, double *W)
     int rc = dalloc(&dU, 2 * (size_t)n);
     if (rc) return rc;
     dW = dU + n;
-    hipError_t e = launch_gather_uw(r->prec, r->state, n, state_layout(r), s, done & 1, dU, dW, r->stream);
+    hipError_t e = launch_gather_uw(r->prec, r->state, r->uw, n, state_layout(r), s, done & 1, dU, dW,
+                                    r->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
     if (e == hipSuccess) e = hipMemcpy(U, dU, sizeof(double) * n, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(W, dW, sizeof(double) * n, hipMemcpyDeviceToHost);

@@ -108,6 +108,11 @@ struct StepArgs {
     // fields of their neighbours; rows at 0 keep no velocity (null in exact mode).
     T *uw;
     const int *pml;
+    // 1: rows of kind 0 (their split fields are gathered by no M-PML row) keep
+    // only U, W (in uw), p_u = p1+p2+p3 and p_w = p4+p5+p6 (in the slots of p1
+    // and p4) and the memory fields: their split updates are linear, so their
+    // sums follow U' = U + p_u dt + alpha (a1+a2+a3) dt^2, p_u' = p_u + (a1+a2+a3) dt
+    int agg;
     size_t fstride;         // n * S
     const int *active;      // active[t] = shots that execute step t (shots sorted)
     // sources grouped by node: entries src_ptr[i]..src_ptr[i+1]-1 of node i hold
@@ -131,9 +136,11 @@ hipError_t launch_step(int precision, int sp, bool coef_per_shot, const void *ar
 hipError_t allow_step_lds(int precision, int sp, bool coef_per_shot, size_t lds_bytes);
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
                              int block, int xcd_remap, int slot, hipStream_t st);
-hipError_t launch_guard(int precision, const void *state, int n, const StateLayout &L,
+// uw (FP32 forms, may be null): U and W are read from it instead of being summed
+// from the split fields
+hipError_t launch_guard(int precision, const void *state, const void *uw, int n, const StateLayout &L,
                         const Ctrl *ctrl, double *energy, int nslots, int every, hipStream_t st);
-hipError_t launch_gather_uw(int precision, const void *state, int n, const StateLayout &L,
+hipError_t launch_gather_uw(int precision, const void *state, const void *uw, int n, const StateLayout &L,
                             int shot, int parity, double *U, double *W, hipStream_t st);
 hipError_t launch_spmv64(int n, const int *rp, const int *cj, const double *a, const double *x,
                          double *y, hipStream_t st);

@@ -228,6 +228,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         if constexpr (!EXACT) pk = ro(a.pml)[row];
         const bool interior = SKIP && pk < 2;    // short gather
         const bool keepv = EXACT || pk != 0;     // velocity stored
+        // row of kind 0 that keeps only the sums U, W, p_u, p_w (see StepArgs::agg)
+        const bool agg = SKIP && a.agg && pk == 0;
         // Kahan terms: 6 fields in the same layout as the state, or (C16) 3
         // words of two bfloat16 terms each
         constexpr int NC = C16 ? 3 : 6;
@@ -241,22 +243,43 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         else co = node_base(row);
         // own-row data nobody else gathers: with CAP it is requested before the
         // gathers so that its latency overlaps theirs
+        // (summed rows: p_u, p_w in the slots of p1 and p4, their Kahan terms in
+        // word 0 as a bfloat16 pair, or in terms 0 and 1; kept here at 0 and 3)
         T pre_acc[6], pre_c[6];
+        auto load_terms = [&]() __attribute__((always_inline)) {
+            if (agg) {
+                if constexpr (C16) {
+                    const uint32_t w = ld_once<NT>(win + co);
+                    pre_c[0] = bf16_lo(w);
+                    pre_c[3] = bf16_hi(w);
+                } else if constexpr (KAHAN) {
+                    pre_c[0] = ld_once<NT>(cin + co);
+                    pre_c[3] = ld_once<NT>(cin + FS + co);
+                }
+            } else {
+                if constexpr (C16) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const uint32_t w = ld_once<NT>(win + k * FS + co);
+                        pre_c[2 * k] = bf16_lo(w);
+                        pre_c[2 * k + 1] = bf16_hi(w);
+                    }
+                } else if constexpr (KAHAN) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) pre_c[q] = ld_once<NT>(cin + q * FS + co);
+                }
+            }
+        };
         if constexpr (PRE) {
             const size_t o0 = node_base(row);
+            if (agg) {
+                pre_acc[0] = ld_once<NT>(in + U1TT * FS + o0);
+                pre_acc[3] = ld_once<NT>(in + W1TT * FS + o0);
+            } else {
 #pragma unroll
-            for (int q = 0; q < 6; ++q) pre_acc[q] = ld_once<NT>(in + (U1TT + q) * FS + o0);
-            if constexpr (C16) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const uint32_t w = ld_once<NT>(win + k * FS + co);
-                    pre_c[2 * k] = bf16_lo(w);
-                    pre_c[2 * k + 1] = bf16_hi(w);
-                }
-            } else if constexpr (KAHAN) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) pre_c[q] = ld_once<NT>(cin + q * FS + co);
+                for (int q = 0; q < 6; ++q) pre_acc[q] = ld_once<NT>(in + (U1TT + q) * FS + o0);
             }
+            load_terms();
         }
 
         // ---- the 32 distinct products over one walk of the row ------------
@@ -267,6 +290,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         const int k0 = ro(a.rp)[row], k1 = ro(a.rp)[row + 1];
         const size_t o = node_base(row);
         T cur[NFIELD];
+        T own_u = 0, own_w = 0;                  // summed rows: the row's own U, W
         bool got = false;
         auto nonzero = [&](int k) __attribute__((always_inline)) {
             const int j = LDSG ? 0 : ro(a.ci)[k];
@@ -348,6 +372,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                 if (j == row) {
                     cur[LX1] = lx1; cur[LX2] = lx2; cur[LX3] = lx3; cur[LX4] = lx4;
                     cur[LY1] = ly1; cur[LY2] = ly2; cur[LY3] = ly3; cur[LY4] = ly4;
+                    own_u = U;
+                    own_w = W;
                     got = true;
                 }
             }
@@ -378,8 +404,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         if (interior) {
             // the row's own displacements are not gathered here
             if constexpr (PRE) {
+                if (!agg) {
 #pragma unroll
-                for (int f = U1; f <= W3; ++f) cur[f] = ld_once<NT>(in + f * FS + o);
+                    for (int f = U1; f <= W3; ++f) cur[f] = ld_once<NT>(in + f * FS + o);
+                }
             }
             walk(nonzero_in, std::integral_constant<int, UN>{});
         } else {
@@ -442,12 +470,18 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                 if (f < U1TT || f > W3TT) cur[f] = x[gathered_slot(f) * GS];
         } else if (interior) {
             if constexpr (!PRE) {
+                if (!agg) {
 #pragma unroll
-                for (int f = U1; f <= W3; ++f) cur[f] = in[f * FS + o];
+                    for (int f = U1; f <= W3; ++f) cur[f] = in[f * FS + o];
+                }
             }
             if (!CAP || !got) {
 #pragma unroll
                 for (int f = LX1; f < NFIELD; ++f) cur[f] = in[f * FS + o];
+                if (agg) {
+                    own_u = uwin[uw_base(row)];
+                    own_w = uwin[uw_base(row) + FS];
+                }
             }
         } else if (!CAP || !got) {
             // (every row of an assembled operator holds its diagonal, so with
@@ -456,8 +490,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             for (int f = 0; f < NFIELD; ++f)
                 if (f < U1TT || f > W3TT) cur[f] = in[f * FS + o];
         }
+        if (agg) {
+            cur[U1TT] = PRE ? pre_acc[0] : ld_once<NT>(in + U1TT * FS + o);
+            cur[W1TT] = PRE ? pre_acc[3] : ld_once<NT>(in + W1TT * FS + o);
+        } else {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) cur[U1TT + q] = PRE ? pre_acc[q] : ld_once<NT>(in + (U1TT + q) * FS + o);
+            for (int q = 0; q < 6; ++q) cur[U1TT + q] = PRE ? pre_acc[q] : ld_once<NT>(in + (U1TT + q) * FS + o);
+        }
 
         // ---- lumped-mass solve with Dirichlet clamp (:681-729) ------------
         T an[6], ln[8];
@@ -494,59 +533,81 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         // ---- Newmark update (:757-776) ---------------------------------------
         const T ca = a.ca, al = a.alpha, cd = a.cd, de = a.delta;
         T dn[6], cn[6];
-        if constexpr (C16 && !PRE) {
+        T sum_u, sum_w;                          // U and W of the new state
+        if constexpr (C16 && !PRE) load_terms();
+        if (agg) {
+            // the three split updates of a summed row, added: U' = U + p_u dt +
+            // alpha (a1 + a2 + a3) dt^2, p_u' = p_u + (a1 + a2 + a3) dt, and for W
+            const T an_s[2] = {an[0] + an[1] + an[2], an[3] + an[4] + an[5]};
+            T res[2];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const uint32_t w = ld_once<NT>(win + k * FS + co);
-                pre_c[2 * k] = bf16_lo(w);
-                pre_c[2 * k + 1] = bf16_hi(w);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const T d = cur[U1 + q];
-            if constexpr (EXACT) {
-                const T v = cur[U1T + q], acc = cur[U1TT + q];
-                dn[q] = d + v * dt + (ca * acc + al * an[q]) * dt * dt;
-                st_once<NT>(out + (U1T + q) * FS + o, v + (cd * acc + de * an[q]) * dt);
-                st_once<NT>(out + (U1TT + q) * FS + o, an[q]);
-            } else {
-                // p-form: the acceleration slot holds p = v + ca dt a
-                const T p = cur[U1TT + q];
+            for (int h = 0; h < 2; ++h) {
+                const T d = h ? own_w : own_u;
+                const T p = cur[U1TT + 3 * h];
                 if constexpr (KAHAN) {
-                    const T inc = p * dt + al * an[q] * dt * dt;
-                    const T y = inc - ((PRE || C16) ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
-                    const T sum = d + y;
-                    cn[q] = (sum - d) - y;
-                    if constexpr (!C16) st_once<NT>(cout + q * FS + co, cn[q]);
-                    dn[q] = sum;
+                    const T inc = p * dt + al * an_s[h] * dt * dt;
+                    const T y = inc - ((PRE || C16) ? pre_c[3 * h] : ld_once<NT>(cin + h * FS + co));
+                    const T sm = d + y;
+                    cn[3 * h] = (sm - d) - y;
+                    if constexpr (!C16) st_once<NT>(cout + h * FS + co, cn[3 * h]);
+                    res[h] = sm;
                 } else {
-                    dn[q] = d + p * dt + al * an[q] * dt * dt;
+                    res[h] = d + p * dt + al * an_s[h] * dt * dt;
                 }
-                if (keepv) st_once<NT>(out + (U1T + q) * FS + o, p + de * an[q] * dt);
-                st_once<NT>(out + (U1TT + q) * FS + o, p + (de + ca) * an[q] * dt);
+                st_once<NT>(out + (U1TT + 3 * h) * FS + o, p + (de + ca) * an_s[h] * dt);
             }
-            st_once<NT>(out + (U1 + q) * FS + o, dn[q]);
-        }
-        if constexpr (C16) {
+            sum_u = res[0];
+            sum_w = res[1];
+            if constexpr (C16) st_once<NT>(wout + co, bf16_bits(cn[0]) | (bf16_bits(cn[3]) << 16));
+        } else {
 #pragma unroll
-            for (int k = 0; k < 3; ++k)
-                st_once<NT>(wout + k * FS + co, bf16_bits(cn[2 * k]) | (bf16_bits(cn[2 * k + 1]) << 16));
+            for (int q = 0; q < 6; ++q) {
+                const T d = cur[U1 + q];
+                if constexpr (EXACT) {
+                    const T v = cur[U1T + q], acc = cur[U1TT + q];
+                    dn[q] = d + v * dt + (ca * acc + al * an[q]) * dt * dt;
+                    st_once<NT>(out + (U1T + q) * FS + o, v + (cd * acc + de * an[q]) * dt);
+                    st_once<NT>(out + (U1TT + q) * FS + o, an[q]);
+                } else {
+                    // p-form: the acceleration slot holds p = v + ca dt a
+                    const T p = cur[U1TT + q];
+                    if constexpr (KAHAN) {
+                        const T inc = p * dt + al * an[q] * dt * dt;
+                        const T y = inc - ((PRE || C16) ? pre_c[q] : ld_once<NT>(cin + q * FS + co));
+                        const T sum = d + y;
+                        cn[q] = (sum - d) - y;
+                        if constexpr (!C16) st_once<NT>(cout + q * FS + co, cn[q]);
+                        dn[q] = sum;
+                    } else {
+                        dn[q] = d + p * dt + al * an[q] * dt * dt;
+                    }
+                    if (keepv) st_once<NT>(out + (U1T + q) * FS + o, p + de * an[q] * dt);
+                    st_once<NT>(out + (U1TT + q) * FS + o, p + (de + ca) * an[q] * dt);
+                }
+                st_once<NT>(out + (U1 + q) * FS + o, dn[q]);
+            }
+            if constexpr (C16) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    st_once<NT>(wout + k * FS + co, bf16_bits(cn[2 * k]) | (bf16_bits(cn[2 * k + 1]) << 16));
+            }
+            // the sums every neighbour forms from the split fields, in the same order
+            sum_u = dn[0] + dn[1] + dn[2];
+            sum_w = dn[3] + dn[4] + dn[5];
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) st_once<NT>(out + (LX1 + q) * FS + o, ln[q]);
         if constexpr (WUW) {
-            // the sums every neighbour forms from the split fields, in the same order
             const size_t ob = uw_base(row);
-            st_once<NT>(uwout + ob, dn[0] + dn[1] + dn[2]);
-            st_once<NT>(uwout + ob + FS, dn[3] + dn[4] + dn[5]);
+            st_once<NT>(uwout + ob, sum_u);
+            st_once<NT>(uwout + ob + FS, sum_w);
         }
 
         // ---- receivers sitting on this node --------------------------------
         const int r0 = ro(a.rec_ptr)[row], r1 = ro(a.rec_ptr)[row + 1];
         if (r0 < r1) {
-            const double Un = (double)(dn[0] + dn[1] + dn[2]);
-            const double Wn = (double)(dn[3] + dn[4] + dn[5]);
+            const double Un = (double)sum_u;
+            const double Wn = (double)sum_w;
             for (int q = r0; q < r1; ++q) {
                 const size_t w = ((size_t)s * a.nsteps_max + t) * a.rec_num + ro(a.rec_idx)[q];
                 a.seis_u[w] = Un;
@@ -737,7 +798,7 @@ hipError_t launch_step(int precision, int sp, bool cps, const void *args, int nb
 // Threads are laid out [rows][shots] (shots innermost, as the state), each
 // accumulates a strided set of nodes, then a block reduction per shot.
 template <class T>
-__global__ __launch_bounds__(256) void guard_kernel(const T *state, int n, const StateLayout L,
+__global__ __launch_bounds__(256) void guard_kernel(const T *state, const T *uw, int n, const StateLayout L,
                                                     const Ctrl *ctrl, double *energy, int nslots,
                                                     int every, int nodes_per_block)
 {
@@ -749,15 +810,25 @@ __global__ __launch_bounds__(256) void guard_kernel(const T *state, int n, const
     const int sl = threadIdx.x % sg, rl = threadIdx.x / sg;
     const int s = blockIdx.y * sg + sl;
     const T *in = state + (size_t)(t & 1) * NFIELD * L.F;
+    // U and W kept beside the state (FP32 forms), in its layout with 2 fields
+    const T *sums = uw ? uw + (size_t)(t & 1) * 2 * L.F : nullptr;
+    StateLayout L2 = L;
+    L2.nf = 2;
     double eu = 0.0, ew = 0.0;
     const int n0 = blockIdx.x * nodes_per_block;
     const int n1 = min(n, n0 + nodes_per_block);
     if (rl < rstep && s < S) {
         for (int i = n0 + rl; i < n1; i += rstep) {
-            const double U = (double)(in[state_index(L, i, U1, s)] + in[state_index(L, i, U2, s)] +
-                                      in[state_index(L, i, U3, s)]);
-            const double W = (double)(in[state_index(L, i, W1, s)] + in[state_index(L, i, W2, s)] +
-                                      in[state_index(L, i, W3, s)]);
+            double U, W;
+            if (sums) {
+                U = (double)sums[state_index(L2, i, 0, s)];
+                W = (double)sums[state_index(L2, i, 1, s)];
+            } else {
+                U = (double)(in[state_index(L, i, U1, s)] + in[state_index(L, i, U2, s)] +
+                             in[state_index(L, i, U3, s)]);
+                W = (double)(in[state_index(L, i, W1, s)] + in[state_index(L, i, W2, s)] +
+                             in[state_index(L, i, W3, s)]);
+            }
             eu += U * U;
             ew += W * W;
         }
@@ -777,28 +848,37 @@ __global__ __launch_bounds__(256) void guard_kernel(const T *state, int n, const
     }
 }
 
-hipError_t launch_guard(int precision, const void *state, int n, const StateLayout &L, const Ctrl *ctrl,
-                        double *energy, int nslots, int every, hipStream_t st)
+hipError_t launch_guard(int precision, const void *state, const void *uw, int n, const StateLayout &L,
+                        const Ctrl *ctrl, double *energy, int nslots, int every, hipStream_t st)
 {
     const int npb = 2048;
     const int sg = L.S < 64 ? L.S : 64;
     const dim3 g((n + npb - 1) / npb, (L.S + sg - 1) / sg);
     if (precision == 0)
-        hipLaunchKernelGGL(guard_kernel<double>, g, dim3(256), 0, st, (const double *)state, n, L,
-                           ctrl, energy, nslots, every, npb);
+        hipLaunchKernelGGL(guard_kernel<double>, g, dim3(256), 0, st, (const double *)state, (const double *)uw,
+                           n, L, ctrl, energy, nslots, every, npb);
     else
-        hipLaunchKernelGGL(guard_kernel<float>, g, dim3(256), 0, st, (const float *)state, n, L,
-                           ctrl, energy, nslots, every, npb);
+        hipLaunchKernelGGL(guard_kernel<float>, g, dim3(256), 0, st, (const float *)state, (const float *)uw,
+                           n, L, ctrl, energy, nslots, every, npb);
     return hipGetLastError();
 }
 
 // ---- U = U1+U2+U3, W = W1+W2+W3 of one shot (snapshots, :775-776, :797-806) --
 template <class T>
-__global__ void gather_uw_kernel(const T *state, int n, const StateLayout L, int shot, int parity,
+__global__ void gather_uw_kernel(const T *state, const T *uw, int n, const StateLayout L, int shot, int parity,
                                  double *U, double *W)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (uw) {
+        // U and W kept beside the state (FP32 forms)
+        const T *sums = uw + (size_t)parity * 2 * L.F;
+        StateLayout L2 = L;
+        L2.nf = 2;
+        U[i] = (double)sums[state_index(L2, i, 0, shot)];
+        W[i] = (double)sums[state_index(L2, i, 1, shot)];
+        return;
+    }
     const T *in = state + (size_t)parity * NFIELD * L.F;
     U[i] = (double)(in[state_index(L, i, U1, shot)] + in[state_index(L, i, U2, shot)] +
                     in[state_index(L, i, U3, shot)]);
@@ -806,16 +886,16 @@ __global__ void gather_uw_kernel(const T *state, int n, const StateLayout L, int
                     in[state_index(L, i, W3, shot)]);
 }
 
-hipError_t launch_gather_uw(int precision, const void *state, int n, const StateLayout &L, int shot,
-                            int parity, double *U, double *W, hipStream_t st)
+hipError_t launch_gather_uw(int precision, const void *state, const void *uw, int n, const StateLayout &L,
+                            int shot, int parity, double *U, double *W, hipStream_t st)
 {
     const dim3 g((n + 255) / 256);
     if (precision == 0)
-        hipLaunchKernelGGL(gather_uw_kernel<double>, g, dim3(256), 0, st, (const double *)state, n, L,
-                           shot, parity, U, W);
+        hipLaunchKernelGGL(gather_uw_kernel<double>, g, dim3(256), 0, st, (const double *)state,
+                           (const double *)uw, n, L, shot, parity, U, W);
     else
-        hipLaunchKernelGGL(gather_uw_kernel<float>, g, dim3(256), 0, st, (const float *)state, n, L,
-                           shot, parity, U, W);
+        hipLaunchKernelGGL(gather_uw_kernel<float>, g, dim3(256), 0, st, (const float *)state,
+                           (const float *)uw, n, L, shot, parity, U, W);
     return hipGetLastError();
 }
 

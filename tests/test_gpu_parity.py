@@ -208,10 +208,10 @@ def test_kernel_variants_agree_bit_for_bit(sp, prec, block):
     assert np.max(np.abs(res[0][0])) > 0
     for v, r in enumerate(res[1:], start=2):
         if v >= 7 and prec != gpu.FP64:
-            # the LDS variant always walks all 20 fields; the FP32 forms of the
-            # others skip terms that are exact zeros outside the damping layers,
-            # which can change the sign of a zero result, never its value
-            assert all(np.array_equal(x, y) for x, y in zip(r, res[0]))
+            # the LDS variant keeps every row's split fields; the FP32 forms of
+            # the others step rows far from the damping layers as sums, which
+            # changes only FP32 rounding
+            assert all(rel_l2(x, y) < 1e-5 for x, y in zip(r, res[0]))
         else:
             assert all(bits_equal(x, y) for x, y in zip(r, res[0]))
     md.close()
@@ -219,13 +219,14 @@ def test_kernel_variants_agree_bit_for_bit(sp, prec, block):
 
 
 @pytest.mark.parametrize("prec", [gpu.FP32, gpu.FP32_KAHAN, gpu.FP32_KAHAN16])
-def test_fp32_interior_rows_short_gather_equals_full_walk(prec):
+def test_fp32_short_gather_and_summed_rows_match_full_walk(prec):
     """FP32 forms: a row outside every damping layer gathers U, W and the
-    memory fields only (10 values per neighbour instead of 20).  The terms it
-    skips enter its equations multiplied by M-PML profiles that are exactly
-    zero, so seismograms at every node and the final wavefield equal those of
-    the full walk (the LDS-staged variant 7, which always gathers all 20) for
-    lane groups of 1, 4, 16, 32 and 64 shots."""
+    memory fields only (10 values per neighbour instead of 20; the terms it
+    skips are exact zeros), and a row that no M-PML row gathers from steps its
+    split fields as sums.  Every lane grouping (1, 4, 16, 32, 64 shots) gives
+    the same seismograms at every node and the same final wavefield, within FP32
+    rounding (1e-5 rel-L2) of the LDS-staged variant 7, which walks all 20 fields
+    and keeps every row's split fields."""
     case = _case(host.Q4, 12, 50, nshots=40)
     prob = case["problem"]
     n = prob["node_num"]
@@ -244,8 +245,9 @@ def test_fp32_interior_rows_short_gather_equals_full_walk(prec):
         res.append(run.seismograms() + (U, W))
         run.close()
     assert np.max(np.abs(res[0][0])) > 0
-    for r in res[1:]:
-        assert all(np.array_equal(x, y) for x, y in zip(r, res[0]))
+    for r in res[2:]:
+        assert all(np.array_equal(x, y) for x, y in zip(r, res[1]))
+    assert all(rel_l2(x, y) < 1e-5 for x, y in zip(res[1], res[0]))
     md.close()
     op.close()
 
