@@ -154,7 +154,8 @@ typedef struct {
                                   (16, 32 or 64 shots per lane group only).  Same results (bit for
                                   bit in FP64; in the FP32 forms variant 7 can differ in the sign
                                   of a zero); a multi-run launch runs 4 to 6 as 5 in the FP32 forms
-                                  and as 4 in FP64, 3 and 7 as 1 */
+                                  and as 4 in FP64, 3 as 3 for 32- and 64-shot groups (else as
+                                  1), 7 as 1 */
     const int *row_order;      /* [node_num] permutation: the order in which the step kernel's
                                   waves take rows (e.g. sfh_locality_order), NULL = node order.
                                   Changes only which rows run together, not any result */

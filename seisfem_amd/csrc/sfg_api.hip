@@ -827,8 +827,10 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.comp = comp;
         a.uw = (decltype(comp))r->uw;
         a.pml = r->pml;
-        // the LDS-staged walk reads every neighbour's split fields, so it keeps them on all rows
-        a.agg = (r->prec != 0 && d->kernel_variant != 7) ? 1 : 0;
+        // the LDS-staged walk reads every neighbour's split fields, so it keeps them on all
+        // rows; one row per lane (lane groups of 1 or 4 shots) mixes row kinds in a wave
+        // and measured slower with summed rows
+        a.agg = (r->prec != 0 && d->kernel_variant != 7 && node_major(r->SP)) ? 1 : 0;
         a.fstride = F;
         a.active = r->active;
         a.src_ptr = r->src;

@@ -663,25 +663,33 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
     const int local = (int)bid - tile_start[r];
     const int bx = local % a.nbx, by = local / a.nbx;
     // Kernel variants of one launch: separate value arrays (0), packed values
-    // (1; also runs 2 and 6), and for 32- and 64-shot groups the form with
-    // own-row data first (runs 3, 4 and 5): 3 nonzeros per trip in the FP32
-    // forms, 2 in FP64.  The launch's register count is that of its largest
-    // form; deeper unrolls would cost every run a wave per SIMD (FP64 with 3
-    // per trip: 256 VGPRs, one wave).
+    // (1; also runs 6), and for 32- and 64-shot groups streaming hints (2) and
+    // the form with own-row data first (runs 3, 4 and 5): 3 nonzeros per trip
+    // in the FP32 forms, 2 in FP64.  The launch's register count is that of
+    // its largest form; deeper unrolls would cost every run a wave per SIMD
+    // (FP64 with 3 per trip: 256 VGPRs, one wave).
     constexpr int D = PREC == 0 ? 3 : 4;
     const int v = a.variant;
-    const int w = v == 0 ? 0 : (v >= 3 && v <= 5) ? 3 : 1;
+    const int w = v == 0 ? 0 : (v >= 3 && v <= 5) ? 3 : v == 2 ? 2 : 1;
 #define SFG_T(SPV, C, VV) step_tile<T, PREC, SPV, C, VV>(a, t, bx, by)
+#define SFG_G(SPV, C)                                  \
+    do {                                               \
+        if (w == 3) SFG_T(SPV, C, D);                  \
+        else if (w == 2) SFG_T(SPV, C, 2);             \
+        else if (w) SFG_T(SPV, C, 1);                  \
+        else SFG_T(SPV, C, 0);                         \
+    } while (0)
     if (a.sp == 1) {
         if (a.cps) { if (w) SFG_T(1, true, 1); else SFG_T(1, true, 0); }
         else { if (w) SFG_T(1, false, 1); else SFG_T(1, false, 0); }
     } else if (a.sp == 32) {
-        if (a.cps) { if (w == 3) SFG_T(32, true, D); else if (w) SFG_T(32, true, 1); else SFG_T(32, true, 0); }
-        else { if (w == 3) SFG_T(32, false, D); else if (w) SFG_T(32, false, 1); else SFG_T(32, false, 0); }
+        if (a.cps) SFG_G(32, true);
+        else SFG_G(32, false);
     } else {
-        if (a.cps) { if (w == 3) SFG_T(64, true, D); else if (w) SFG_T(64, true, 1); else SFG_T(64, true, 0); }
-        else { if (w == 3) SFG_T(64, false, D); else if (w) SFG_T(64, false, 1); else SFG_T(64, false, 0); }
+        if (a.cps) SFG_G(64, true);
+        else SFG_G(64, false);
     }
+#undef SFG_G
 #undef SFG_T
 }
 

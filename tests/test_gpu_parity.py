@@ -222,11 +222,11 @@ def test_kernel_variants_agree_bit_for_bit(sp, prec, block):
 def test_fp32_short_gather_and_summed_rows_match_full_walk(prec):
     """FP32 forms: a row outside every damping layer gathers U, W and the
     memory fields only (10 values per neighbour instead of 20; the terms it
-    skips are exact zeros), and a row that no M-PML row gathers from steps its
-    split fields as sums.  Every lane grouping (1, 4, 16, 32, 64 shots) gives
-    the same seismograms at every node and the same final wavefield, within FP32
-    rounding (1e-5 rel-L2) of the LDS-staged variant 7, which walks all 20 fields
-    and keeps every row's split fields."""
+    skips are exact zeros), so lane groups of 1 and 4 shots give the numbers of
+    the LDS-staged variant 7, which walks all 20 fields.  With 16, 32 and 64
+    shots a row that no M-PML row gathers from also steps its split fields as
+    sums: those give one another's numbers, within FP32 rounding (1e-5 rel-L2)
+    of the split form.  Seismograms at every node and the final wavefield."""
     case = _case(host.Q4, 12, 50, nshots=40)
     prob = case["problem"]
     n = prob["node_num"]
@@ -236,8 +236,9 @@ def test_fp32_short_gather_and_summed_rows_match_full_walk(prec):
     op = gpu.Operator(n, prob["csr"])
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
     res = []
-    for sp, v, block in [(64, 7, 64), (64, 5, 64), (64, 1, 256), (32, 6, 128), (16, 5, 64), (4, 4, 128),
-                         (1, 1, 64)]:
+    split_form = [(64, 7, 64), (4, 4, 128), (1, 1, 64)]
+    summed = [(64, 5, 64), (64, 1, 256), (64, 3, 256), (32, 6, 128), (16, 5, 64)]
+    for sp, v, block in split_form + summed:
         run = gpu.Run(op, md, case["src"], 50, np.arange(n, dtype=np.int32), case["dt"], case["f0"], case["t0"],
                       precision=prec, shots_per_lane_group=sp, guard_every=25, kernel_variant=v, block=block)
         run.advance(50)
@@ -245,9 +246,12 @@ def test_fp32_short_gather_and_summed_rows_match_full_walk(prec):
         res.append(run.seismograms() + (U, W))
         run.close()
     assert np.max(np.abs(res[0][0])) > 0
-    for r in res[2:]:
-        assert all(np.array_equal(x, y) for x, y in zip(r, res[1]))
-    assert all(rel_l2(x, y) < 1e-5 for x, y in zip(res[1], res[0]))
+    a, b = res[:len(split_form)], res[len(split_form):]
+    for r in a[1:]:
+        assert all(np.array_equal(x, y) for x, y in zip(r, a[0]))
+    for r in b[1:]:
+        assert all(np.array_equal(x, y) for x, y in zip(r, b[0]))
+    assert all(rel_l2(x, y) < 1e-5 for x, y in zip(b[0], a[0]))
     md.close()
     op.close()
 
@@ -310,8 +314,10 @@ def test_per_shot_media_match_their_own_oracle_runs():
 def test_per_shot_media_fp32_forms_within_tolerance():
     """Per-shot media in the FP32 form with bfloat16 Kahan terms (per-shot
     M-PML profiles decide the short gather and the rows that keep a velocity):
-    every shot within rel-L2 1e-4 of the FP64 run of the same media, and the
-    same numbers from 4-shot field-major and 64-shot node-major lane groups."""
+    every shot within rel-L2 1e-4 of the FP64 run of the same media, and
+    4-shot field-major and 64-shot node-major lane groups (split fields kept
+    on every row, or stepped as sums far from the damping layers) within FP32
+    rounding of each other."""
     nelem, steps = 14, 120
     models = [(2000.0, 3000.0), (2300.0, 3400.0), (1800.0, 2600.0)]
     cases = []
@@ -340,7 +346,7 @@ def test_per_shot_media_fp32_forms_within_tolerance():
         res.append(run.seismograms())
         run.close()
     (u64, w64), (ua, wa), (ub, wb) = res
-    assert np.array_equal(ua, ub) and np.array_equal(wa, wb)
+    assert rel_l2(ub, ua) < 1e-5 and rel_l2(wb, wa) < 1e-5
     for k in range(len(cases)):
         a = np.concatenate([ua[k].ravel(), wa[k].ravel()])
         b = np.concatenate([u64[k].ravel(), w64[k].ravel()])
