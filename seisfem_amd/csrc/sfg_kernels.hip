@@ -133,7 +133,8 @@ __device__ __forceinline__ int begin_step(Ctrl *ctrl, int slot)
 // zero in the same expressions, so the results equal those of the full walk.
 // The FP32 forms also update displacements with the p-form of the Newmark
 // step (NEWMARK_ALPHA): a row far from the damping layers reads and writes
-// displacements, p and memory fields only.
+// displacements, p and memory fields only; and with a.agg a row that no M-PML
+// row gathers from keeps its split fields as sums (StepArgs::agg).
 template <class T, int PREC, int SP, bool CPS, int V>
 __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, int by)
 {
