@@ -431,7 +431,8 @@ def test_multi_run_fp32_forms_match_separate_runs():
 def test_distributed_sources():
     """(1) a point force given as a one-entry distributed source is bit-identical
     to the default path; (2) an explosive source on the symmetry line of a
-    homogeneous square gives u even and w odd about that line."""
+    homogeneous square gives u even and w odd about that line; (3) the same
+    source in the batched FP32 form stays within 1e-4 rel-L2 of it."""
     nelem = 14
     case = workloads.structured_case(
         host.Q4, nelem, nelem, 10.0,
@@ -465,7 +466,16 @@ def test_distributed_sources():
     assert su > 0
     assert np.max(np.abs(u2[0, -1] - u2[0, -1][mirror])) < 1e-9 * su
     assert np.max(np.abs(w2[0, -1] + w2[0, -1][mirror])) < 1e-9 * max(su, np.max(np.abs(w2)))
-    for r in (r0, r1, r2):
+    # the same explosive source in the batched FP32 form (64-shot lane group: short
+    # gather, summed rows), within the FP32 bar of the FP64 run
+    r3 = gpu.Run(op, md, case["src"], 80, allnodes, case["dt"], case["f0"], case["t0"], sources=[(nodes, wts)],
+                 precision=gpu.FP32_KAHAN16, shots_per_lane_group=64, kernel_variant=3)
+    r3.advance(80)
+    u3, w3 = r3.seismograms()
+    a = np.concatenate([u3.ravel(), w3.ravel()])
+    b = np.concatenate([u2.ravel(), w2.ravel()])
+    assert rel_l2(a, b) < 1e-4, rel_l2(a, b)
+    for r in (r0, r1, r2, r3):
         r.close()
     md.close()
     op.close()
