@@ -256,6 +256,34 @@ def test_fp32_short_gather_and_summed_rows_match_full_walk(prec):
     op.close()
 
 
+def test_fp32_energy_and_snapshots_track_fp64():
+    """The energy guard's sums and the snapshots of the FP32 form read U and W
+    kept beside the state (summed rows keep no split fields): with 70 shots in
+    64-shot groups they match the FP64 run's within 1e-4 (relative per energy
+    record, rel-L2 per snapshot)."""
+    case = _case(host.Q4, 16, 100, nshots=70)
+    prob = case["problem"]
+    n = prob["node_num"]
+    say("energies and snapshots: Q4 16x16, 70 shots, FP32 + bfloat16 Kahan terms vs FP64")
+    op = gpu.Operator(n, prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    out = []
+    for prec in (gpu.FP64, gpu.FP32_KAHAN16):
+        run = gpu.Run(op, md, case["src"], 100, case["rec"], case["dt"], case["f0"], case["t0"], precision=prec,
+                      shots_per_lane_group=64, kernel_variant=3, guard_every=20)
+        run.advance(100)
+        out.append((run.energy(), [run.snapshot(k) for k in (0, 64, 69)]))
+        run.close()
+    (e64, s64), (e32, s32) = out
+    assert e64.shape == e32.shape and e64.shape[0] >= 5
+    big = e64 > 1e-12 * np.max(e64)
+    assert np.any(big) and np.max(np.abs(e32[big] - e64[big]) / e64[big]) < 1e-4
+    for (ua, wa), (ub, wb) in zip(s32, s64):
+        assert rel_l2(np.concatenate([ua, wa]), np.concatenate([ub, wb])) < 1e-4
+    md.close()
+    op.close()
+
+
 def test_lds_variant_rejects_single_shot_groups():
     case = _case(host.Q4, 8, 10)
     prob = case["problem"]
