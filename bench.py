@@ -12,6 +12,7 @@ before the timed region.
                (HIP events on the kernel's stream), against 8 TB/s
   fp32_kahan_terms  the same job with the compensation terms in FP32
   fp64_single  the plain single-shot FP64 loop (bit-identical to the reference) on the same mesh
+  fp64_batched the same job in FP64 (bit-identical per shot), the accuracy check's reference
   accuracy     max rel-L2 (u and w concatenated) of the FP32 seismograms vs the FP64 loop
   cpu_baseline the C restatement of the reference loop (oracle) on the host cores, bounded sample
 
@@ -186,13 +187,21 @@ def main():
 
     # accuracy gate: FP32(+Kahan) seismograms vs the FP64 loop, full length, a few shots
     acc = None
+    fp64_batched = None
     if not args.no_acc:
         nacc = args.acc_shots if 0 < args.acc_shots < args.shots else args.shots
         idx = np.unique(np.linspace(0, args.shots - 1, nacc).astype(int))
-        ra = gpu.Run(op, md, case["src"][idx], K, case["rec"], precision=gpu.FP64, nsteps=nsteps[idx], **base)
+        # the reference's arithmetic for all these shots in one run, same launch shape as the
+        # batched job (every variant and tile map gives the same bits)
+        ra = gpu.Run(op, md, case["src"][idx], K, case["rec"], precision=gpu.FP64, nsteps=nsteps[idx], **kw)
         ra.advance(K)
+        ms_a = ra.last_ms_per_step
         u64, w64 = ra.seismograms()
         ra.close()
+        dof_a = 2.0 * N * float(np.minimum(nsteps[idx], K).sum())
+        fp64_batched = {"value": dof_a / (ms_a * K * 1e-3), "ms_per_step": ms_a, "shots": int(len(idx))}
+        log(f"batched fp64 (reference arithmetic, {len(idx)} shots): {ms_a:.3f} ms/step, "
+            f"{fp64_batched['value'] / 1e9:.2f} G DOF-steps/s")
         def shot_errors(su, sw):
             errs = []
             for q, s in enumerate(idx):
@@ -335,6 +344,7 @@ def main():
                                           "achieved": bytes_full / (ms_all_active * 1e-3) / 1e9}},
         "fp64_single": {"value": fp64_rate, "ms_per_step": ms64, "speedup_batched_vs_fp64": value / fp64_rate},
         "fp32_kahan_terms": None if k32 is None else dict(k32, speedup_vs_fp64=k32["value"] / fp64_rate),
+        "fp64_batched": fp64_batched,
         "accuracy_max_rel_l2": acc,
         "multi_model": multi,
         "cpu_baseline": cpu,
