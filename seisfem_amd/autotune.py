@@ -48,29 +48,43 @@ def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None
     orders = {"node": None}
     if coords is not None:
         orders["hilbert"] = host.locality_order(*coords)
+    def timed(sp, block, variant, xcd, oname, k):
+        # two graph replays of k launches: the first warms up, the second is timed
+        run = gpu.Run(op, md, src, 2 * k, rec, dt, f0, t0, precision=precision, block=block,
+                      shots_per_lane_group=sp, kernel_variant=variant, xcd_remap=xcd, guard_every=k,
+                      row_order=orders[oname])
+        run.advance(k)
+        run.advance(k)
+        ms = run.last_ms_per_step
+        run.close()
+        return ms
+
     table = []
     for sp, block, variant, xcd, oname in (cands or candidates(nshots)):
         if oname not in orders:
             continue
-        # two graph replays of `steps` launches: the first warms up, the second is timed
         try:
-            run = gpu.Run(op, md, src, 2 * steps, rec, dt, f0, t0, precision=precision, block=block,
-                          shots_per_lane_group=sp, kernel_variant=variant, xcd_remap=xcd, guard_every=steps,
-                          row_order=orders[oname])
+            ms = timed(sp, block, variant, xcd, oname, steps)
         except gpu.NativeError as e:
             if e.code != -1:  # only an argument the shape cannot take (e.g. a halo beyond LDS) is skipped
                 raise
             if log:
                 log(f"tune skip sp={sp} block={block} variant={variant} order={oname}: {e}")
             continue
-        run.advance(steps)
-        run.advance(steps)
-        ms = run.last_ms_per_step
-        run.close()
         cand = dict(shots_per_lane_group=sp, block=block, kernel_variant=variant, xcd_remap=xcd, row_order=oname)
         table.append((cand, ms))
         if log:
             log(f"tune {cand}: {ms:.4f} ms/step")
+    # the few fastest are often within a per cent: time them again over twice the
+    # steps and keep the better of the two figures of each
+    table.sort(key=lambda c: c[1])
+    for i in range(min(3, len(table))):
+        c, ms = table[i]
+        ms2 = timed(c["shots_per_lane_group"], c["block"], c["kernel_variant"], c["xcd_remap"], c["row_order"],
+                    2 * steps)
+        table[i] = (c, min(ms, ms2))
+        if log:
+            log(f"tune again {c}: {ms2:.4f} ms/step")
     best = dict(min(table, key=lambda c: c[1])[0])
     best["row_order"] = orders[best["row_order"]]
     _cache[key] = (best, table)
