@@ -10,13 +10,22 @@ run's stream; the fastest candidate wins.  Results are cached per (nodes, nnz,
 shots, precision).  Every candidate gives the same results bit for bit.
 """
 import itertools
+import math
 
 from . import gpu, host
 
 _cache = {}
 
 
-def candidates(nshots):
+def stripe_widths(nodes, sp, block):
+    """XCD stripe widths to try: about (row blocks per grid line) / 8, the line
+    length taken as sqrt(nodes) (a roughly square mesh), halved and doubled."""
+    rows_per_block = (block // 64) * (1 if sp >= 64 else 64 // sp)
+    q = math.sqrt(max(nodes, 1)) / (rows_per_block * 8.0)
+    return sorted({max(1, int(round(q * f))) for f in (0.5, 1.0, 2.0)})
+
+
+def candidates(nshots, nodes=160801):
     """(shots per lane group, block, kernel variant, xcd remap, row order) to try."""
     if nshots <= 1:
         sps, blocks = [1], [64, 128]
@@ -34,7 +43,9 @@ def candidates(nshots):
     if nshots > 1:
         # XCD stripes of Q row blocks (node order): all XCDs sweep the rows together;
         # Q near (row blocks per grid line) / 8 keeps vertical neighbours in one L2
-        cands += list(itertools.product(sps, blocks, variants, [6, 12, 25], ["node"]))
+        for sp, block in itertools.product(sps, blocks):
+            qs = [q for q in stripe_widths(nodes, sp, block) if q > 1]
+            cands += list(itertools.product([sp], [block], variants, qs, ["node"]))
     return cands
 
 
@@ -60,7 +71,7 @@ def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None
         return ms
 
     table = []
-    for sp, block, variant, xcd, oname in (cands or candidates(nshots)):
+    for sp, block, variant, xcd, oname in (cands or candidates(nshots, op.node_num)):
         if oname not in orders:
             continue
         try:
