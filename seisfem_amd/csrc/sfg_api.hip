@@ -394,6 +394,7 @@ struct sfg_run {
     int *pml = nullptr;         // FP32 forms: rows with a nonzero M-PML profile
     Ctrl *ctrl = nullptr;
     int *active = nullptr, *src = nullptr, *rec_ptr = nullptr, *rec_idx = nullptr;
+    int *shot_steps = nullptr;  // [S] executed steps per internal shot (0 for padding)
     int *order = nullptr;       // row_order on the device (or null)
     int *hptr = nullptr, *hnode = nullptr, *kslot = nullptr, *rslot = nullptr; // LDS halos
     size_t lds_bytes = 0;       // dynamic LDS of the LDS-staged kernel
@@ -423,6 +424,7 @@ static void run_free(sfg_run *r)
     dfree(r->pml);
     dfree(r->ctrl);
     dfree(r->active);
+    dfree(r->shot_steps);
     dfree(r->order);
     dfree(r->hptr);
     dfree(r->hnode);
@@ -472,8 +474,8 @@ static StateLayout state_layout(const sfg_run *r)
 // ctrl / stream default to the run's own (a multi-run passes its shared ones)
 static int launch_guard_now(sfg_run *r, const Ctrl *ctrl = nullptr, hipStream_t st = nullptr)
 {
-    HIPCHK(launch_guard(r->prec, r->state, r->uw, r->op->n, state_layout(r), ctrl ? ctrl : r->ctrl, r->energy,
-                        r->nslots, r->guard_every, st ? st : r->stream));
+    HIPCHK(launch_guard(r->prec, r->state, r->uw, r->op->n, state_layout(r), ctrl ? ctrl : r->ctrl, r->shot_steps,
+                        r->energy, r->nslots, r->guard_every, st ? st : r->stream));
     return SFG_OK;
 }
 
@@ -718,7 +720,8 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         (r->prec >= 2 && (rc = dalloc((char **)&r->comp, 2 * comp_fields(r->prec) * F * w))) ||
         (r->prec != 0 && ((rc = dalloc((char **)&r->uw, 2 * 2 * F * w)) || (rc = dalloc(&r->pml, n)))) ||
         (rc = dalloc(&r->ctrl, 1)) ||
-        (rc = dalloc(&r->active, d->max_steps)) || (rc = dalloc(&r->src, n + 1)) ||
+        (rc = dalloc(&r->active, d->max_steps)) || (rc = dalloc(&r->shot_steps, r->S)) ||
+        (rc = dalloc(&r->src, n + 1)) ||
         (rc = dalloc(&r->src_shot, sshot.size())) || (rc = dalloc(&r->src_wt, swt.size())) ||
         (rc = dalloc(&r->rec_ptr, n + 1)) || (rc = dalloc(&r->rec_idx, ridx.size())) ||
         (rc = dalloc(&r->wavelet, d->max_steps)) || (rc = dalloc(&r->seis_u, seis_n)) ||
@@ -739,7 +742,11 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (e == hipSuccess) e = launch_fill_zero(r->seis_u, seis_n * 8, r->stream);
     if (e == hipSuccess) e = launch_fill_zero(r->seis_w, seis_n * 8, r->stream);
     if (e == hipSuccess) e = launch_fill_zero(r->energy, (size_t)r->nslots * r->S * 16, r->stream);
+    std::vector<int> shot_steps(r->S, 0);
+    std::copy(r->steps_int.begin(), r->steps_int.end(), shot_steps.begin());
     if (e == hipSuccess) e = hipMemcpyAsync(r->active, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(r->shot_steps, shot_steps.data(), sizeof(int) * r->S, hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess && r->order)
         e = hipMemcpyAsync(r->order, d->row_order, sizeof(int) * n, hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess && r->hptr) e = hipMemcpyAsync(r->hptr, hptr.data(), sizeof(int) * hptr.size(), hipMemcpyHostToDevice, r->stream);

@@ -137,9 +137,11 @@ hipError_t allow_step_lds(int precision, int sp, bool coef_per_shot, size_t lds_
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
                              int block, int xcd_remap, int slot, hipStream_t st);
 // uw (FP32 forms, may be null): U and W are read from it instead of being summed
-// from the split fields
+// from the split fields.  shot_steps[s] (s < L.S): steps shot s executes; a shot
+// that has finished is read from the buffer of its own last step
 hipError_t launch_guard(int precision, const void *state, const void *uw, int n, const StateLayout &L,
-                        const Ctrl *ctrl, double *energy, int nslots, int every, hipStream_t st);
+                        const Ctrl *ctrl, const int *shot_steps, double *energy, int nslots, int every,
+                        hipStream_t st);
 hipError_t launch_gather_uw(int precision, const void *state, const void *uw, int n, const StateLayout &L,
                             int shot, int parity, double *U, double *W, hipStream_t st);
 hipError_t launch_spmv64(int n, const int *rp, const int *cj, const double *a, const double *x,

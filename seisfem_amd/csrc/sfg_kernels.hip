@@ -808,8 +808,8 @@ hipError_t launch_step(int precision, int sp, bool cps, const void *args, int nb
 // accumulates a strided set of nodes, then a block reduction per shot.
 template <class T>
 __global__ __launch_bounds__(256) void guard_kernel(const T *state, const T *uw, int n, const StateLayout L,
-                                                    const Ctrl *ctrl, double *energy, int nslots,
-                                                    int every, int nodes_per_block)
+                                                    const Ctrl *ctrl, const int *shot_steps, double *energy,
+                                                    int nslots, int every, int nodes_per_block)
 {
     __shared__ double red_u[256], red_w[256];
     const int t = executed_steps(ctrl);
@@ -818,9 +818,11 @@ __global__ __launch_bounds__(256) void guard_kernel(const T *state, const T *uw,
     const int rstep = blockDim.x / sg;
     const int sl = threadIdx.x % sg, rl = threadIdx.x / sg;
     const int s = blockIdx.y * sg + sl;
-    const T *in = state + (size_t)(t & 1) * NFIELD * L.F;
+    // a shot that stopped after k < t steps keeps its last state in buffer k & 1
+    const int done = s < S ? min(t, shot_steps[s]) : t;
+    const T *in = state + (size_t)(done & 1) * NFIELD * L.F;
     // U and W kept beside the state (FP32 forms), in its layout with 2 fields
-    const T *sums = uw ? uw + (size_t)(t & 1) * 2 * L.F : nullptr;
+    const T *sums = uw ? uw + (size_t)(done & 1) * 2 * L.F : nullptr;
     StateLayout L2 = L;
     L2.nf = 2;
     double eu = 0.0, ew = 0.0;
@@ -858,17 +860,18 @@ __global__ __launch_bounds__(256) void guard_kernel(const T *state, const T *uw,
 }
 
 hipError_t launch_guard(int precision, const void *state, const void *uw, int n, const StateLayout &L,
-                        const Ctrl *ctrl, double *energy, int nslots, int every, hipStream_t st)
+                        const Ctrl *ctrl, const int *shot_steps, double *energy, int nslots, int every,
+                        hipStream_t st)
 {
     const int npb = 2048;
     const int sg = L.S < 64 ? L.S : 64;
     const dim3 g((n + npb - 1) / npb, (L.S + sg - 1) / sg);
     if (precision == 0)
         hipLaunchKernelGGL(guard_kernel<double>, g, dim3(256), 0, st, (const double *)state, (const double *)uw,
-                           n, L, ctrl, energy, nslots, every, npb);
+                           n, L, ctrl, shot_steps, energy, nslots, every, npb);
     else
         hipLaunchKernelGGL(guard_kernel<float>, g, dim3(256), 0, st, (const float *)state, (const float *)uw,
-                           n, L, ctrl, energy, nslots, every, npb);
+                           n, L, ctrl, shot_steps, energy, nslots, every, npb);
     return hipGetLastError();
 }
 

@@ -284,6 +284,41 @@ def test_fp32_energy_and_snapshots_track_fp64():
     op.close()
 
 
+@pytest.mark.parametrize("sp", [1, 4, 64])
+def test_energy_records_of_uneven_shots_match_the_oracle(sp):
+    """Energy-guard records with uneven step counts: a shot that stopped after
+    k steps is summed from the buffer of its own last step, so every record of
+    check c equals the oracle's sum(U^2), sum(W^2) after min(c * every, k)
+    steps (elastic_wave.c:777-785; rel 1e-12: the device reduction adds in
+    another order).  Step counts of both parities finish between checks."""
+    case = _case(host.Q4, 14, 65, nshots=5)
+    prob = case["problem"]
+    nsteps = np.array([65, 48, 33, 57, 20], np.int32)
+    every = 10
+    say(f"energy records, uneven steps {list(nsteps)}, FP64, sp={sp}, guard every {every}")
+    op = gpu.Operator(prob["node_num"], prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    run = gpu.Run(op, md, case["src"], 65, case["rec"], case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
+                  nsteps=nsteps, guard_every=every, shots_per_lane_group=sp)
+    run.advance(65)
+    e = run.energy()
+    run.close()
+    md.close()
+    op.close()
+    # guards run after steps 10, 20, ..., 60 and 65 (checks 1 .. 7)
+    checks = {c: min(c * every, 65) for c in range(1, e.shape[0])}
+    assert max(checks) == 7
+    for s in range(5):
+        k = int(nsteps[s])
+        ref = oracle.oracle_masslump(prob, case["src"][s], k + 2, case["rec"], case["dt"], case["f0"], case["t0"])
+        assert ref["status"] == k
+        for c, t in checks.items():
+            j = min(t, k)                      # executed steps of this shot at the check
+            want = ref["energy"][j + 1]        # row it = j + 1 holds the sums after step j
+            got = e[c, s]
+            assert want[0] > 0 and np.all(np.abs(got - want) <= 1e-12 * want), (s, c, got, want)
+
+
 def test_lds_variant_rejects_single_shot_groups():
     case = _case(host.Q4, 8, 10)
     prob = case["problem"]
