@@ -173,6 +173,11 @@ int sfg_run_create_media(sfg_run **run, const sfg_operator *op, const sfg_medium
 /* Executes up to nsteps further steps (fewer at the end).  Returns the number
  * of steps executed so far (>= 0) or an error code. */
 int sfg_run_advance(sfg_run *run, int nsteps);
+/* Returns the run to its state at creation without reallocating: every field,
+ * seismogram and energy record zeroed and the step counter at 0, so the same
+ * shots can be stepped again (queued on the run's stream, no host wait).
+ * Not for a run stepped by a multi-run. */
+int sfg_run_reset(sfg_run *run);
 /* Waits for queued work; returns the executed step count. */
 int sfg_run_sync(sfg_run *run);
 /* Receiver displacement U_now/W_now[rec_node] per executed step

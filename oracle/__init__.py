@@ -52,6 +52,8 @@ def olib():
             C.POINTER(_pd), _i, _d, _d, _d, _d, _d, _i, _i, _pi, _pd, _pd, _i, _pd, _pd, _pi, _pd,
             _pd, _pi]
         L.oracle_elastic_masslump.restype = _i
+        L.oracle_elastic_masslump_timed.argtypes = L.oracle_elastic_masslump.argtypes + [C.c_char_p, _pd]
+        L.oracle_elastic_masslump_timed.restype = _i
         _olib = L
     return _olib
 
@@ -210,10 +212,12 @@ def oracle_csr_matvec(p, j, x_, v):
 
 
 def oracle_masslump(prob, src_node, step, rec_nodes, dt, f0, t0, amplitude=1.0e10, angle_deg=90.0,
-                    snap_every=0, want_state=False):
+                    snap_every=0, want_state=False, io_dir=None):
     """One shot of the reference loop (step = the reference's ``step``; it
     executes step-2 steps).  Returns dict(seis_u, seis_w, snaps_u, snaps_w,
-    energy, state, status)."""
+    energy, state, status, loop_seconds).  With io_dir the loop also writes
+    the reference's %f seismogram and wavefield text files there;
+    loop_seconds is omp_get_wtime() around the step loop alone."""
     L = olib()
     n = prob["node_num"]
     csr = [(np.ascontiguousarray(p, np.int32), np.ascontiguousarray(j, np.int32),
@@ -238,11 +242,13 @@ def oracle_masslump(prob, src_node, step, rec_nodes, dt, f0, t0, amplitude=1.0e1
     energy = np.zeros((step, 2))
     state = np.zeros((26, n)) if want_state else None
     zm = C.c_int(0)
-    rc = L.oracle_elastic_masslump(
+    loop_s = C.c_double(0.0)
+    rc = L.oracle_elastic_masslump_timed(
         n, P, J, X, Cc, _dp(mass), _ip(flag), Pm, step, dt, f0, t0, amplitude, angle_deg, int(src_node),
         rec.size, _ip(rec) if rec.size else _pi(), _dp(su), _dp(sw), int(snap_every),
         _dp(snu) if snap_every > 0 else _pd(), _dp(snw) if snap_every > 0 else _pd(), C.byref(nsnap),
-        _dp(energy), _dp(state) if want_state else _pd(), C.byref(zm))
+        _dp(energy), _dp(state) if want_state else _pd(), C.byref(zm),
+        None if io_dir is None else os.fsencode(io_dir), C.byref(loop_s))
     k = nsnap.value
     return dict(seis_u=su, seis_w=sw, snaps_u=snu[:k], snaps_w=snw[:k], energy=energy, state=state,
-                status=rc, zero_mass=zm.value)
+                status=rc, zero_mass=zm.value, loop_seconds=loop_s.value)

@@ -1,0 +1,104 @@
+"""ORACLE -- TEST INFRASTRUCTURE ONLY: the CPU-baseline leg of bench.py.
+
+Times the C restatement of the reference's masslump loop (sfem_oracle.c,
+time_evolution/elastic_wave.c:446-809 with the 44 csr_matvec calls per step of
+:533-580 and :599-646) on the host cores, FP64, one shot of the benchmark's
+mesh, as the reference runs it.  The step loop alone is timed with
+omp_get_wtime(), once without output and once with the reference's %f text
+output (seismogram rows every step, wavefield rows every 200 steps,
+:787-806) written to a temporary directory.
+
+Run by bench.py in a child process that touches no GPU, so that the OpenMP
+runtime starts with OMP_NUM_THREADS = the physical cores it may use,
+OMP_PROC_BIND=close and OMP_PLACES=cores:
+
+    python -m oracle.cpu_baseline --nelem 400 --seconds 15
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+
+def physical_cores():
+    """(physical cores, logical CPUs) among the CPUs this process may run on."""
+    allowed = sorted(os.sched_getaffinity(0))
+    cores = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            blocks = f.read().strip().split("\n\n")
+        info = {}
+        for b in blocks:
+            d = dict((k.strip(), v.strip()) for k, v in (ln.split(":", 1) for ln in b.splitlines() if ":" in ln))
+            if "processor" in d:
+                info[int(d["processor"])] = (d.get("physical id", "0"), d.get("core id", d["processor"]))
+        for c in allowed:
+            cores.add(info.get(c, ("0", str(c))))
+    except (OSError, ValueError):
+        cores = set(allowed)
+    return len(cores), len(allowed)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next(ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        return "unknown CPU"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nelem", type=int, default=400)
+    ap.add_argument("--seconds", type=float, default=15.0, help="host time of both timed loops together")
+    ap.add_argument("--max-steps", type=int, default=3000)
+    args = ap.parse_args()
+    # before the OpenMP runtime starts: with OMP_PROC_BIND set it binds this thread to one place
+    phys, logical = physical_cores()
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import oracle
+    from seisfem_amd import workloads
+
+    t_setup = time.time()
+    # the benchmark's mesh and model; M-PML profiles from the oracle (no GPU in this process)
+    case = workloads.config4a(nshots=1, nelem=args.nelem, mpml_fn=oracle.oracle_abc_mpml)
+    prob = case["problem"]
+    setup_s = time.time() - t_setup
+    src = int(case["src"][0])
+
+    def loop(k, io_dir=None):
+        # a unit force: the loop's work does not depend on the amplitude, and the reference's
+        # 1e10 can trip the energy guard (sum U^2 > 1e7) on small meshes, which stops the loop
+        r = oracle.oracle_masslump(prob, src, k + 2, case["rec"], case["dt"], case["f0"], case["t0"],
+                                   amplitude=1.0, io_dir=io_dir)
+        assert r["status"] == k, r["status"]
+        return r["loop_seconds"]
+
+    # calibration, then two loops of about half the budget each (whole shots when they fit)
+    per_step = loop(10) / 10.0
+    k = int(min(args.max_steps, max(200, 0.5 * args.seconds / max(per_step, 1e-9))))
+    t_noio = loop(k)
+    with tempfile.TemporaryDirectory() as d:
+        t_io = loop(k, io_dir=d)
+        io_bytes = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+    n = prob["node_num"]
+    out = {
+        "threads": int(os.environ.get("OMP_NUM_THREADS", "0") or 0),
+        "physical_cores": phys, "logical_cpus": logical, "cpu_model": cpu_model(),
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
+        "nodes": n, "dof": 2 * n, "steps": k, "rec_num": int(len(case["rec"])),
+        "loop_s_no_io": t_noio, "loop_s_with_io": t_io, "io_bytes": io_bytes,
+        "dof_steps_per_s_no_io": 2.0 * n * k / t_noio, "dof_steps_per_s_with_io": 2.0 * n * k / t_io,
+        "ms_per_step_no_io": 1e3 * t_noio / k, "ms_per_step_with_io": 1e3 * t_io / k, "setup_s": setup_s,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -16,7 +16,10 @@
  *   oracle_elastic_masslump  time_evolution/elastic_wave.c:446-809 (one shot,
  *                            solver "masslump"), with the 44 csr_matvec calls
  *                            of :533-580 and :599-646 on the seven separate
- *                            CSR matrices exactly as the reference issues them.
+ *                            CSR matrices exactly as the reference issues them;
+ *                            oracle_elastic_masslump_timed is the same loop
+ *                            with the reference's %f text output (:787-806)
+ *                            and a timer around the step loop (CPU baseline).
  *
  * Deviations (documented, none changes a finite result):
  *   - the energy guard (:779-785) returns ORACLE_E_ENERGY instead of exit(1);
@@ -30,6 +33,8 @@
  * invariants (zero state stays zero, no-PML reduction, linearity in the source).
  */
 #include <math.h>
+#include <omp.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -167,15 +172,23 @@ enum { NSTATE = 26 };
  * the row count.  energy: [step x 2] (NULL to skip).  final_state: NSTATE x
  * node_num (NULL to skip).  Returns the executed step count (step-2) or a
  * negative code; *zero_mass_count receives the number of zero-mass events.
+ *
+ * oracle_elastic_masslump_timed adds the reference's text output and a timer:
+ * with io_dir set it writes io_dir/seismogram_{u,w}.txt ("%f   " per receiver,
+ * one row per step, elastic_wave.c:787-795) and io_dir/wavefield_{u,w}.txt
+ * ("%f\t" per node: the zero row, then every step with (it+1) % 200 == 0,
+ * :497-507, :797-806); *loop_seconds receives omp_get_wtime() around the step
+ * loop alone (the arrays are allocated and zeroed before it starts).
  */
-int oracle_elastic_masslump(int node_num, const int *const csr_p[7], const int *const csr_j[7],
-                            const double *const csr_x[7], const double *const c[4],
-                            const double *mass_lump, const int *dirichlet,
-                            const double *const pml[6], int step, double dt, double f0, double t0,
-                            double amplitude, double angle_deg, int source_node, int rec_num,
-                            const int *rec_node, double *seis_u, double *seis_w, int snap_every,
-                            double *snap_u, double *snap_w, int *nsnap, double *energy,
-                            double *final_state, int *zero_mass_count)
+int oracle_elastic_masslump_timed(int node_num, const int *const csr_p[7], const int *const csr_j[7],
+                                  const double *const csr_x[7], const double *const c[4],
+                                  const double *mass_lump, const int *dirichlet,
+                                  const double *const pml[6], int step, double dt, double f0, double t0,
+                                  double amplitude, double angle_deg, int source_node, int rec_num,
+                                  const int *rec_node, double *seis_u, double *seis_w, int snap_every,
+                                  double *snap_u, double *snap_w, int *nsnap, double *energy,
+                                  double *final_state, int *zero_mass_count, const char *io_dir,
+                                  double *loop_seconds)
 {
     if (node_num <= 0 || step < 2) return ORACLE_E_ARG;
     const int N = node_num, csr_p_size = node_num + 1;
@@ -195,6 +208,36 @@ int oracle_elastic_masslump(int node_num, const int *const csr_p[7], const int *
     if (!st || !tt_new || !UW || !eq || !rhs) {
         free(st); free(tt_new); free(UW); free(eq); free(rhs);
         return ORACLE_E_NOMEM;
+    }
+    /* touch every page before the timed loop, as the reference's zeroing loop
+       over its arrays does (:466-502); calloc maps them lazily */
+    memset(st, 0, sizeof(double) * NSTATE * (size_t)N);
+    memset(tt_new, 0, sizeof(double) * 6 * (size_t)N);
+    memset(UW, 0, sizeof(double) * 2 * (size_t)N);
+    memset(eq, 0, sizeof(double) * 22 * (size_t)N);
+    memset(rhs, 0, sizeof(double) * 14 * (size_t)N);
+    /* the reference's output files, opened before the loop as it does (:451-463) */
+    FILE *fsu = NULL, *fsw = NULL, *fwu = NULL, *fww = NULL;
+    if (io_dir) {
+        char path[4096];
+        snprintf(path, sizeof path, "%s/seismogram_u.txt", io_dir); fsu = fopen(path, "w");
+        snprintf(path, sizeof path, "%s/seismogram_w.txt", io_dir); fsw = fopen(path, "w");
+        snprintf(path, sizeof path, "%s/wavefield_u.txt", io_dir); fwu = fopen(path, "w");
+        snprintf(path, sizeof path, "%s/wavefield_w.txt", io_dir); fww = fopen(path, "w");
+        if (!fsu || !fsw || !fwu || !fww) {
+            if (fsu) fclose(fsu);
+            if (fsw) fclose(fsw);
+            if (fwu) fclose(fwu);
+            if (fww) fclose(fww);
+            free(st); free(tt_new); free(UW); free(eq); free(rhs);
+            return ORACLE_E_ARG;
+        }
+        for (int i = 0; i < N; i++) { /* the zero row, :497-502 */
+            fprintf(fwu, "%f\t", UW[i]);
+            fprintf(fww, "%f\t", UW[N + i]);
+        }
+        fprintf(fwu, "\n");
+        fprintf(fww, "\n");
     }
 #define F(k) (st + (size_t)(k) * N)
     double *U1 = F(0), *U2 = F(1), *U3 = F(2), *W1 = F(3), *W2 = F(4), *W3 = F(5);
@@ -226,6 +269,7 @@ int oracle_elastic_masslump(int node_num, const int *const csr_p[7], const int *
         memset(snap_w, 0, sizeof(double) * N);
         ns = 1;
     }
+    const double wt0 = omp_get_wtime();
     for (int it = 2; it < step; it++) {
         const double time = (it + 1) * dt;
         double Eu = 0.0, Ew = 0.0;
@@ -330,6 +374,22 @@ int oracle_elastic_masslump(int node_num, const int *const csr_p[7], const int *
                 seis_w[(size_t)(it - 2) * rec_num + r] = W[rec_node[r]];
             }
         }
+        if (fsu) { /* :787-806 */
+            for (int r = 0; r < rec_num; r++) {
+                fprintf(fsu, "%f   ", U[rec_node[r]]);
+                fprintf(fsw, "%f   ", W[rec_node[r]]);
+            }
+            fprintf(fsu, "\n");
+            fprintf(fsw, "\n");
+            if ((it + 1) % 200 == 0) {
+                for (int q = 0; q < N; q++) {
+                    fprintf(fwu, "%f\t", U[q]);
+                    fprintf(fww, "%f\t", W[q]);
+                }
+                fprintf(fwu, "\n");
+                fprintf(fww, "\n");
+            }
+        }
         if (snap_u && snap_every > 0 && (it + 1) % snap_every == 0) {
             memcpy(snap_u + (size_t)ns * N, U, sizeof(double) * N);
             memcpy(snap_w + (size_t)ns * N, W, sizeof(double) * N);
@@ -342,9 +402,31 @@ int oracle_elastic_masslump(int node_num, const int *const csr_p[7], const int *
     }
 #undef MV
 #undef MM
+    if (loop_seconds) *loop_seconds = omp_get_wtime() - wt0;
+    if (fsu) {
+        fclose(fsu);
+        fclose(fsw);
+        fclose(fwu);
+        fclose(fww);
+    }
     if (nsnap) *nsnap = ns;
     if (zero_mass_count) *zero_mass_count = zero_mass;
     if (final_state) memcpy(final_state, st, sizeof(double) * NSTATE * (size_t)N);
     free(st); free(tt_new); free(UW); free(eq); free(rhs);
     return status ? status : step - 2;
+}
+
+int oracle_elastic_masslump(int node_num, const int *const csr_p[7], const int *const csr_j[7],
+                            const double *const csr_x[7], const double *const c[4],
+                            const double *mass_lump, const int *dirichlet,
+                            const double *const pml[6], int step, double dt, double f0, double t0,
+                            double amplitude, double angle_deg, int source_node, int rec_num,
+                            const int *rec_node, double *seis_u, double *seis_w, int snap_every,
+                            double *snap_u, double *snap_w, int *nsnap, double *energy,
+                            double *final_state, int *zero_mass_count)
+{
+    return oracle_elastic_masslump_timed(node_num, csr_p, csr_j, csr_x, c, mass_lump, dirichlet, pml, step, dt,
+                                         f0, t0, amplitude, angle_deg, source_node, rec_num, rec_node, seis_u,
+                                         seis_w, snap_every, snap_u, snap_w, nsnap, energy, final_state,
+                                         zero_mass_count, NULL, NULL);
 }

@@ -77,6 +77,7 @@ def lib():
         "sfg_run_create_media": (_i, [C.POINTER(_vp), _vp, C.POINTER(_vp), C.POINTER(ShotDesc)]),
         "sfg_run_advance": (_i, [_vp, _i]),
         "sfg_run_sync": (_i, [_vp]),
+        "sfg_run_reset": (_i, [_vp]),
         "sfg_run_seismograms": (_i, [_vp, _pd, _pd]),
         "sfg_run_snapshot": (_i, [_vp, _i, _pd, _pd]),
         "sfg_run_energy": (_i, [_vp, _pd, _i]),

@@ -951,6 +951,27 @@ extern "C" int sfg_run_advance(sfg_run *r, int nsteps)
     return r->executed;
 }
 
+extern "C" int sfg_run_reset(sfg_run *r)
+{
+    g_err.clear();
+    if (!r) return fail(SFG_E_ARG, "reset: null run");
+    if (r->in_multi) return fail(SFG_E_ARG, "reset: this run is stepped by a multi-run");
+    const size_t F = (size_t)r->op->n * r->S;
+    const size_t w = r->prec == 0 ? 8 : 4;
+    const size_t seis_n = (size_t)r->S * r->max_steps * std::max(r->rec_num, 1);
+    // the same zero fills as at creation (elastic_wave.c:466-502), on the run's stream
+    HIPCHK(launch_fill_zero(r->state, 2 * NFIELD * F * w, r->stream));
+    if (r->comp) HIPCHK(launch_fill_zero(r->comp, 2 * comp_fields(r->prec) * F * w, r->stream));
+    if (r->uw) HIPCHK(launch_fill_zero(r->uw, 2 * 2 * F * w, r->stream));
+    HIPCHK(launch_fill_zero(r->ctrl, sizeof(Ctrl), r->stream));
+    HIPCHK(launch_fill_zero(r->seis_u, seis_n * 8, r->stream));
+    HIPCHK(launch_fill_zero(r->seis_w, seis_n * 8, r->stream));
+    HIPCHK(launch_fill_zero(r->energy, (size_t)r->nslots * r->S * 16, r->stream));
+    r->executed = 0;
+    r->last_ms = 0.0;
+    return SFG_OK;
+}
+
 extern "C" int sfg_run_sync(sfg_run *r)
 {
     if (!r) return fail(SFG_E_ARG, "sync: null run");

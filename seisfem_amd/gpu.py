@@ -143,6 +143,10 @@ class Run:
     def sync(self):
         return check(lib().sfg_run_sync(self.h), "sfg_run_sync")
 
+    def reset(self):
+        """Back to the state at creation (zero fields, seismograms, step 0)."""
+        return check(lib().sfg_run_reset(self.h), "sfg_run_reset")
+
     def seismograms(self):
         """(u, w) arrays [nshots, max_steps, rec_num]."""
         shape = (self.nshots, self.max_steps, self.rec.size)
