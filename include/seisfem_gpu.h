@@ -186,6 +186,17 @@ int sfg_run_seismograms(sfg_run *run, double *u, double *w);
 /* Current U = U1+U2+U3 and W = W1+W2+W3 of one shot (snapshot rows,
  * elastic_wave.c:797-806), host arrays [node_num]. */
 int sfg_run_snapshot(sfg_run *run, int shot, double *U, double *W);
+/* sfg_run_snapshot without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  U and W are staged in
+ * a fixed device array of 2 * SFG_NOALLOC_MAX_NODES doubles (64 MiB, one copy
+ * per GPU, held until the process ends); runs on meshes with more nodes get
+ * SFG_E_ARG.  Every call on one GPU shares that array: calls from one process
+ * are serialised by the library, so calls that overlap on a GPU only wait for
+ * each other.  The run must live on the calling thread's current device.
+ * Same values as sfg_run_snapshot, bit for bit. */
+#define SFG_NOALLOC_MAX_NODES (1 << 22)
+int sfg_run_snapshot_noalloc(sfg_run *run, int shot, double *U, double *W);
 /* Energy guard values sum(U^2), sum(W^2) per check: [nchecks][nshots][2]. */
 int sfg_run_energy(sfg_run *run, double *energy, int max_checks);
 /* Average device time per step kernel (ms) measured with HIP events on the

@@ -80,6 +80,7 @@ def lib():
         "sfg_run_reset": (_i, [_vp]),
         "sfg_run_seismograms": (_i, [_vp, _pd, _pd]),
         "sfg_run_snapshot": (_i, [_vp, _i, _pd, _pd]),
+        "sfg_run_snapshot_noalloc": (_i, [_vp, _i, _pd, _pd]),
         "sfg_run_energy": (_i, [_vp, _pd, _i]),
         "sfg_run_last_ms_per_step": (_d, [_vp]),
         "sfg_run_step_bytes": (_d, [_vp, _i]),

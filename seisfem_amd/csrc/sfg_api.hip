@@ -1032,6 +1032,55 @@ extern "C" int sfg_run_snapshot(sfg_run *r, int shot, double *U, double *W)
     return SFG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// allocation-free snapshots
+
+// Staging vector of sfg_run_snapshot_noalloc: U then W of one shot, one copy
+// per GPU (module-scope device memory, held until the process ends).  The
+// array's name is a host-side stand-in; host code uses only the address
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[2 * (size_t)SFG_NOALLOC_MAX_NODES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;               // serialises the calls of this process
+double *g_noalloc_ptr[kMaxDevices] = {}; // device address of `scratch`, per device
+} // namespace
+
+extern "C" int sfg_run_snapshot_noalloc(sfg_run *r, int shot, double *U, double *W)
+{
+    g_err.clear();
+    if (!r || !U || !W || shot < 0 || shot >= r->nshots) return fail(SFG_E_ARG, "snapshot_noalloc: bad arguments");
+    const int n = r->op->n;
+    if (n > SFG_NOALLOC_MAX_NODES)
+        return fail(SFG_E_ARG, "snapshot_noalloc: %d nodes exceed the staging limit of %d (use sfg_run_snapshot)", n,
+                    SFG_NOALLOC_MAX_NODES);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(SFG_E_ARG, "snapshot_noalloc: device %d out of range", dev);
+    if (dev != r->op->device)
+        return fail(SFG_E_ARG, "snapshot_noalloc: run lives on device %d, current device is %d", r->op->device, dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        // first call on this device: resolve this GPU's copy of the array
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(SFG_E_HIP, "snapshot_noalloc: no device address for the staging array");
+        g_noalloc_ptr[dev] = (double *)p;
+    }
+    double *dU = g_noalloc_ptr[dev], *dW = dU + n;
+    int s = 0;
+    while (r->perm[s] != shot) ++s;
+    // a shot that stopped early holds its state in the buffer of its own last step
+    const int done = std::min(r->executed, r->steps_int[s]);
+    // every one of the 2n staged values is written by the gather: nothing to clear
+    HIPCHK(launch_gather_uw(r->prec, r->state, r->uw, n, state_layout(r), s, done & 1, dU, dW, r->stream));
+    HIPCHK(hipMemcpyAsync(U, dU, sizeof(double) * n, hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipMemcpyAsync(W, dW, sizeof(double) * n, hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return SFG_OK;
+}
+
 extern "C" int sfg_run_energy(sfg_run *r, double *energy, int max_checks)
 {
     g_err.clear();
