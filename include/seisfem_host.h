@@ -73,6 +73,18 @@ int sfh_lumped_mass(int type_code, int node_num, int element_num, const int *ele
 int sfh_coo2csr(int nnz_coo, int node_num, const int *ai, const int *aj, const double *ax,
                 int *bp, int *bj, double *bx, int *csr_size);
 
+/* Drop-in for the reference's Fortran module procedure under its gfortran
+ * symbol (sparse_matrix/coo2csr_lib.f90:6-41, called by pointer at
+ * time_evolution/elastic_wave.c:241-313), so a caller without a Fortran
+ * compiler links this library instead of coo2csr_lib.o.  Same arguments and
+ * effects: clears Bp (Bp_size entries) and Bj / Bx (nnz entries), adds 1 to
+ * Ai and Aj in place, writes the canonical CSR 0-based (the sfh_coo2csr
+ * result, bit for bit) with the unused tail of Bj at -1 and of Bx at 0, and
+ * sets *csr_size.  A row index outside 0..*Bp_size-2 (where the reference
+ * writes out of bounds) prints to stderr and sets *csr_size = -1. */
+void __coo2csr_lib_MOD_coo2csr_canonical(int *nnz, int *Bp_size, int *csr_size, int *Ai, int *Aj,
+                                          double *Ax, int *Bp, int *Bj, double *Bx);
+
 /* Node search: the LAST node within edge_size/2 in x and y, or -1.
  * Ref: source_receiver/node_location.c:1-28 */
 int sfh_node_location(int node_num, double edge_size, const double *node_x,

@@ -53,6 +53,7 @@ def lib():
         "sfh_assemble_coo": (_i, [_i, _i, _i, _pi, _pd, _pd, _pd, _pi, _pi, C.POINTER(_pd)]),
         "sfh_lumped_mass": (_i, [_i, _i, _i, _pi, _pd, _pd, _pd, _pd]),
         "sfh_coo2csr": (_i, [_i, _i, _pi, _pi, _pd, _pi, _pi, _pd, _pi]),
+        "__coo2csr_lib_MOD_coo2csr_canonical": (None, [_pi, _pi, _pi, _pi, _pi, _pd, _pi, _pi, _pd]),
         "sfh_node_location": (_i, [_i, _d, _pd, _pd, _d, _d]),
         "sfh_station_line": (None, [_i, _d, _d, _d, _d, _pd, _pd]),
         "sfh_dirichlet_flags": (None, [_i, _pd, _pd, _d, _d, _d, _d, _d, _i, _pi]),

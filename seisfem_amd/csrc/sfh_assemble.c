@@ -18,6 +18,7 @@
  * Build with -ffp-contract=off and without -ffast-math.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include "seisfem_host.h"
@@ -452,4 +453,43 @@ int sfh_csr_union(int node_num, int nmat, const int *const *bp, const int *const
         up[i + 1] = w;
     }
     return w;
+}
+
+/* The reference's Fortran entry point under its gfortran name, for callers
+ * linked against coo2csr_lib.o (elastic_wave.c:241-313 call it seven times).
+ * Everything by pointer, as Fortran passes it; semantics of
+ * coo2csr_lib.f90:6-41: the outputs are cleared (Bp, and Bj / Bx over all
+ * nnz entries), Ai and Aj are incremented in place and stay so, the canonical
+ * CSR of the 0-based COO input is written 0-based, *csr_size receives the
+ * kept count, and the unused tail of Bj reads -1 (cleared to 0, then shifted
+ * to 0-based with the rest) while that of Bx reads 0.  The reference has no
+ * error path; here a row index outside 0..Bp_size-2 or a failed allocation
+ * prints a message to stderr and sets *csr_size = -1, with Ai / Aj untouched.
+ * The reference's progress lines (write(*,*)) are not printed. */
+void __coo2csr_lib_MOD_coo2csr_canonical(int *nnz, int *Bp_size, int *csr_size, int *Ai, int *Aj,
+                                          double *Ax, int *Bp, int *Bj, double *Bx)
+{
+    const int n = *nnz, np1 = *Bp_size;
+    for (int i = 0; i < np1; ++i) Bp[i] = 0;
+    for (int k = 0; k < n; ++k) {
+        Bj[k] = 0;
+        Bx[k] = 0.0;
+    }
+    int kept = 0;
+    const int rc = sfh_coo2csr(n, np1 - 1, Ai, Aj, Ax, Bp, Bj, Bx, &kept);
+    if (rc != SFH_OK) {
+        fprintf(stderr, "coo2csr_canonical: %s (nnz %d, Bp_size %d)\n",
+                rc == SFH_E_NOMEM ? "out of memory" : "row index out of range", n, np1);
+        *csr_size = -1;
+        return;
+    }
+    for (int k = kept; k < n; ++k) {
+        Bj[k] = -1;
+        Bx[k] = 0.0;
+    }
+    for (int k = 0; k < n; ++k) {
+        Ai[k] += 1;
+        Aj[k] += 1;
+    }
+    *csr_size = kept;
 }

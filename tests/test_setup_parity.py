@@ -188,3 +188,89 @@ def test_locality_order_is_a_hilbert_walk():
     xr, yr = rng.uniform(-5.0, 7.0, 1000), rng.uniform(100.0, 101.0, 1000)
     o = host.locality_order(xr, yr)
     assert np.array_equal(np.sort(o), np.arange(1000))
+
+
+def _fortran_abi(fn, n, ai, aj, ax):
+    """Calls a coo2csr_canonical with the Fortran ABI (every argument by
+    pointer); returns every output in full (Bp, Bj, Bx over all nnz entries,
+    csr_size) and the Ai / Aj it leaves behind."""
+    import ctypes as C
+    ai = np.array(ai, np.int32)
+    aj = np.array(aj, np.int32)
+    ax = np.array(ax, np.float64)
+    nnz = C.c_int(ai.size)
+    psz = C.c_int(n + 1)
+    csz = C.c_int(-7)
+    bp = np.full(n + 1, 12345, np.int32)
+    bj = np.full(max(ai.size, 1), 777, np.int32)
+    bx = np.full(max(ai.size, 1), np.nan)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+
+    def go():
+        fn(C.byref(nnz), C.byref(psz), C.byref(csz), ip(ai), ip(aj), dp(ax), ip(bp), ip(bj), dp(bx))
+        return csz.value
+
+    k = oracle._big_stack(go)
+    return dict(bp=bp, bj=bj[:ai.size], bx=bx[:ai.size], csr_size=k, ai=ai, aj=aj, ax=ax)
+
+
+def _kat():
+    ai = np.array([0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 2, 2, 2, 1, 1, 1], np.int32)
+    aj = np.array([0, 1, 2, 0, 1, 2, 0, 1, 2, 3, 2, 1, 3, 2, 1, 3, 2, 1], np.int32)
+    ax = np.array([1.0, -0.5, -0.5, -0.5, 0.5, 0.0, -0.5, 0.0, 0.5, 1.0, -0.5, -0.5, -0.5, 0.5, 0.0,
+                   -0.5, 0.0, 0.5])
+    return ai, aj, ax
+
+
+def test_fortran_abi_coo2csr_known_answer():
+    """The library's __coo2csr_lib_MOD_coo2csr_canonical on the known-answer
+    input of sparse_matrix/test_coo2csr_lib.c:10-12: the canonical CSR, the
+    tail of Bj at -1 and of Bx at 0 (coo2csr_lib.f90:23-25, 36-37), and Ai / Aj
+    incremented in place (:26-27)."""
+    from seisfem_amd import lib
+    ai, aj, ax = _kat()
+    out = _fortran_abi(lib().__coo2csr_lib_MOD_coo2csr_canonical, 4, ai, aj, ax)
+    assert out["csr_size"] == 12
+    assert out["bp"].tolist() == [0, 3, 6, 9, 12]
+    assert out["bj"].tolist() == [0, 1, 2, 0, 1, 3, 0, 2, 3, 1, 2, 3] + [-1] * 6
+    assert out["bx"].tolist() == [1.0, -0.5, -0.5, -0.5, 1.0, -0.5, -0.5, 1.0, -0.5, -0.5, -0.5, 1.0] + [0.0] * 6
+    assert np.array_equal(out["ai"], ai + 1) and np.array_equal(out["aj"], aj + 1)
+    assert _bits_equal(out["ax"], ax)
+
+
+def test_fortran_abi_coo2csr_rejects_rows_out_of_range():
+    from seisfem_amd import lib
+    ai, aj, ax = _kat()
+    ai[4] = 4                                          # row 4 of a 4-row matrix
+    out = _fortran_abi(lib().__coo2csr_lib_MOD_coo2csr_canonical, 4, ai, aj, ax)
+    assert out["csr_size"] == -1
+    assert np.array_equal(out["ai"], ai) and np.array_equal(out["aj"], aj)
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["kat", "sorts", "q9", "t6"])
+def test_fortran_abi_coo2csr_matches_reference_fortran(case):
+    """Every output of the library's Fortran-ABI entry point, including the
+    tails of Bj / Bx and the mutated Ai / Aj, equals the reference's own
+    coo2csr_lib.f90 compiled unchanged (oracle/_ref), bit for bit."""
+    from seisfem_amd import lib
+    if case == "kat":
+        n, (ai, aj, ax) = 4, _kat()
+    elif case == "sorts":
+        rng = np.random.default_rng(11)
+        n = 30
+        k = 900
+        ai = rng.integers(0, n, k).astype(np.int32)
+        aj = rng.integers(0, n, k).astype(np.int32)
+        ax = rng.choice([0.0, -0.0, 1.0, -1.0, 0.25, 1e-300], k)
+    else:
+        tc = host.Q9 if case == "q9" else host.T6
+        en, x, y = _mesh(tc, 6, 5)
+        ci, cj, vals = host.assemble_coo(tc, en, x, y, _rho(x.size))
+        n, ai, aj, ax = x.size, ci, cj, vals[3]
+    mine = _fortran_abi(lib().__coo2csr_lib_MOD_coo2csr_canonical, n, ai, aj, ax)
+    theirs = _fortran_abi(oracle.rlib()._QMcoo2csr_libPcoo2csr_canonical, n, ai, aj, ax)
+    assert mine["csr_size"] == theirs["csr_size"] > 0
+    for key in ("bp", "bj", "bx", "ai", "aj", "ax"):
+        assert _bits_equal(mine[key], theirs[key]), key
