@@ -8,7 +8,8 @@ as bfloat16 -- the batched multi-shot configuration.  Inputs are resident in
 HBM before the timed region.
 
 One bench step is one pass of the time loop over the whole job: all 128 shots
-from rest to their own last step (2 998 step launches, one per time step).
+from rest to their own last step (one step launch per time step, 2 995 with
+the seeded step counts).
 --warmup W untimed passes, then --steps K timed passes; the run is reset to
 rest (sfg_run_reset) at the start of every pass.
 
@@ -83,6 +84,42 @@ def cpu_baseline(nelem, seconds):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def config2_leg(args):
+    """Configuration 2 (Q9 400x400 VTI, one shot): FP64 (reference arithmetic) and
+    FP32+Kahan step kernels, each at its tuned (or --launch2) launch shape."""
+    from seisfem_amd import autotune, gpu, workloads
+    t_c = time.time()
+    c2 = workloads.config2(nelem=args.nelem, steps_exec=args.config2_steps)
+    p2 = c2["problem"]
+    n2 = p2["node_num"]
+    op2 = gpu.Operator(n2, p2["csr"])
+    md2 = gpu.Medium(op2, p2["c"], p2["mass_lump"], p2["dirichlet"], p2["mpml"])
+    log(f"config 2 setup {time.time() - t_c:.1f}s: Q9 {args.nelem}x{args.nelem}, N={n2}, nnz={op2.nnz}")
+    out = {"workload": f"Q9 {args.nelem}x{args.nelem} VTI, 1 shot, first {args.config2_steps} steps",
+           "nodes": n2, "nnz_union": op2.nnz}
+    b2 = dict(dt=c2["dt"], f0=c2["f0"], t0=c2["t0"])
+    for name, pr in (("fp64", gpu.FP64), ("fp32_kahan", gpu.FP32_KAHAN)):
+        if args.launch2:
+            v = [int(q) for q in args.launch2.split(",")]
+            best2 = dict(shots_per_lane_group=v[0], block=v[1], kernel_variant=v[2],
+                         xcd_remap=v[3] if v[3] > 1 else bool(v[3]), row_order=None)
+        else:
+            best2, _ = autotune.tune(op2, md2, c2["src"], c2["rec"], precision=pr, steps=20, log=log, **b2)
+        r2 = gpu.Run(op2, md2, c2["src"], args.config2_steps, c2["rec"], precision=pr, **dict(b2, **best2))
+        r2.advance(args.config2_steps)
+        ms2 = r2.last_ms_per_step
+        by2 = r2.step_bytes(1)
+        r2.close()
+        out[name] = {"ms_per_launch": ms2, "dof_steps_per_s": 2.0 * n2 / (ms2 * 1e-3),
+                     "bytes_per_launch": by2, "achieved_GBs": by2 / (ms2 * 1e-3) / 1e9,
+                     "frac": by2 / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, "tuned": autotune.describe(best2)}
+        log(f"config 2 {name}: {ms2:.4f} ms/launch, {out[name]['dof_steps_per_s'] / 1e9:.2f} G DOF-steps/s, "
+            f"{out[name]['achieved_GBs']:.0f} GB/s ({out[name]['frac']:.2f} of peak)")
+    md2.close()
+    op2.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +140,10 @@ def main():
     ap.add_argument("--no-multi", action="store_true", help="skip configuration 4 part 2 (16 models, same launch)")
     ap.add_argument("--no-config2", action="store_true", help="skip the Q9 400x400 single-shot leg")
     ap.add_argument("--config2-steps", type=int, default=400)
+    ap.add_argument("--config2-only", action="store_true",
+                    help="run only the Q9 400x400 leg and print its JSON (profiling runs)")
+    ap.add_argument("--launch2", default="",
+                    help="pin the config-2 launch of both forms, 'shots_per_lane_group,block,variant,xcd' (no tuning)")
     ap.add_argument("--quick", action="store_true", help="small mesh and step counts (smoke of the bench itself)")
     ap.add_argument("--launch", default="",
                     help="pin the batched launch 'shots_per_lane_group,block,variant,xcd[,hilbert]' (no tuning); "
@@ -114,6 +155,12 @@ def main():
               "shots would shard across GPUs with no communication, but no multi-GPU figure is produced.",
               file=sys.stderr)
         sys.exit(2)
+    if args.config2_only:
+        from seisfem_amd import gpu
+        # the copy kernel whose byte counters calibrate the profiler's (FETCH_SIZE, WRITE_SIZE)
+        copy_gbs, _ = gpu.stream_copy(1 << 30, 5)
+        print(json.dumps({"config2": config2_leg(args), "measured_copy_GBs": copy_gbs}), flush=True)
+        return
     step_range = STEP_RANGE
     if args.quick:
         args.nelem, args.fp64_steps, args.cpu_seconds, args.config2_steps = 100, 100, 2.0, 50
@@ -324,32 +371,7 @@ def main():
             h.close()
 
     # configuration 2: Q9 400x400 VTI, one shot, FP64 (reference arithmetic) and FP32+Kahan
-    config2 = None
-    if not args.no_config2:
-        t_c = time.time()
-        c2 = workloads.config2(nelem=args.nelem, steps_exec=args.config2_steps)
-        p2 = c2["problem"]
-        n2 = p2["node_num"]
-        op2 = gpu.Operator(n2, p2["csr"])
-        md2 = gpu.Medium(op2, p2["c"], p2["mass_lump"], p2["dirichlet"], p2["mpml"])
-        log(f"config 2 setup {time.time() - t_c:.1f}s: Q9 {args.nelem}x{args.nelem}, N={n2}, nnz={op2.nnz}")
-        config2 = {"workload": f"Q9 {args.nelem}x{args.nelem} VTI, 1 shot, first {args.config2_steps} steps",
-                   "nodes": n2, "nnz_union": op2.nnz}
-        b2 = dict(dt=c2["dt"], f0=c2["f0"], t0=c2["t0"])
-        for name, pr in (("fp64", gpu.FP64), ("fp32_kahan", gpu.FP32_KAHAN)):
-            best2, _ = autotune.tune(op2, md2, c2["src"], c2["rec"], precision=pr, steps=20, log=log, **b2)
-            r2 = gpu.Run(op2, md2, c2["src"], args.config2_steps, c2["rec"], precision=pr, **dict(b2, **best2))
-            r2.advance(args.config2_steps)
-            ms2 = r2.last_ms_per_step
-            by2 = r2.step_bytes(1)
-            r2.close()
-            config2[name] = {"ms_per_launch": ms2, "dof_steps_per_s": 2.0 * n2 / (ms2 * 1e-3),
-                             "bytes_per_launch": by2, "achieved_GBs": by2 / (ms2 * 1e-3) / 1e9,
-                             "frac": by2 / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, "tuned": autotune.describe(best2)}
-            log(f"config 2 {name}: {ms2:.4f} ms/launch, {config2[name]['dof_steps_per_s'] / 1e9:.2f} G DOF-steps/s, "
-                f"{config2[name]['achieved_GBs']:.0f} GB/s ({config2[name]['frac']:.2f} of peak)")
-        md2.close()
-        op2.close()
+    config2 = None if args.no_config2 else config2_leg(args)
 
     # CPU baseline: the oracle (C restatement of the reference loop), bounded sample, child process
     cpu = None
