@@ -84,6 +84,25 @@ def cpu_baseline(nelem, seconds):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def pinned_launch(spec, x, y):
+    """gpu.Run keywords of a pinned launch 'shots_per_lane_group,block,variant,xcd[,order[,window]]'
+    (xcd: 1 contiguous ranges, 0 round-robin, Q > 1 stripes of Q row blocks; order: 0 node,
+    1 Hilbert, or patchWxH; window: the sliced layouts' sorting window)."""
+    from seisfem_amd import host
+    v = spec.split(",")
+    order, name = None, "node"
+    if len(v) > 4 and v[4] not in ("", "0"):
+        if v[4].startswith("patch"):
+            w, h = (int(q) for q in v[4][5:].split("x"))
+            order, name = host.patch_order(x, y, w, h), v[4]
+        else:
+            order, name = host.locality_order(x, y), "hilbert"
+    xcd = int(v[3])
+    return dict(shots_per_lane_group=int(v[0]), block=int(v[1]), kernel_variant=int(v[2]),
+                xcd_remap=xcd if xcd > 1 else bool(xcd), row_order=order, row_order_name=name,
+                sell_sigma=int(v[5]) if len(v) > 5 else 0)
+
+
 def config2_leg(args):
     """Configuration 2 (Q9 400x400 VTI, one shot): FP64 (reference arithmetic) and
     FP32+Kahan step kernels, each at its tuned (or --launch2) launch shape."""
@@ -100,12 +119,12 @@ def config2_leg(args):
     b2 = dict(dt=c2["dt"], f0=c2["f0"], t0=c2["t0"])
     for name, pr in (("fp64", gpu.FP64), ("fp32_kahan", gpu.FP32_KAHAN)):
         if args.launch2:
-            v = [int(q) for q in args.launch2.split(",")]
-            best2 = dict(shots_per_lane_group=v[0], block=v[1], kernel_variant=v[2],
-                         xcd_remap=v[3] if v[3] > 1 else bool(v[3]), row_order=None)
+            best2 = pinned_launch(args.launch2, p2["x"], p2["y"])
         else:
-            best2, _ = autotune.tune(op2, md2, c2["src"], c2["rec"], precision=pr, steps=20, log=log, **b2)
-        r2 = gpu.Run(op2, md2, c2["src"], args.config2_steps, c2["rec"], precision=pr, **dict(b2, **best2))
+            best2, _ = autotune.tune(op2, md2, c2["src"], c2["rec"], precision=pr, steps=20, log=log,
+                                     coords=(p2["x"], p2["y"]), **b2)
+        r2 = gpu.Run(op2, md2, c2["src"], args.config2_steps, c2["rec"], precision=pr,
+                     **dict(b2, **autotune.run_kwargs(best2)))
         r2.advance(args.config2_steps)
         ms2 = r2.last_ms_per_step
         by2 = r2.step_bytes(1)
@@ -143,11 +162,12 @@ def main():
     ap.add_argument("--config2-only", action="store_true",
                     help="run only the Q9 400x400 leg and print its JSON (profiling runs)")
     ap.add_argument("--launch2", default="",
-                    help="pin the config-2 launch of both forms, 'shots_per_lane_group,block,variant,xcd' (no tuning)")
+                    help="pin the config-2 launch of both forms (same form as --launch; no tuning)")
     ap.add_argument("--quick", action="store_true", help="small mesh and step counts (smoke of the bench itself)")
     ap.add_argument("--launch", default="",
-                    help="pin the batched launch 'shots_per_lane_group,block,variant,xcd[,hilbert]' (no tuning); "
-                         "xcd: 1 contiguous ranges, 0 round-robin, Q > 1 stripes of Q row blocks")
+                    help="pin the batched launch 'shots_per_lane_group,block,variant,xcd[,order[,window]]' (no "
+                         "tuning); xcd: 1 contiguous ranges, 0 round-robin, Q > 1 stripes of Q row blocks; order: "
+                         "0 node, 1 Hilbert, patchWxH patches of W nodes x H grid lines")
     ap.add_argument("--launch64", default="", help="pin the FP64 single-shot launch shape (same form)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -190,27 +210,20 @@ def main():
     base = dict(dt=case["dt"], f0=case["f0"], t0=case["t0"])
     # launch shape chosen by measurement on this problem (untimed)
     coords = (prob["x"], prob["y"])
-    hilbert = host.locality_order(*coords)
-
-    def pinned(spec):
-        v = [int(q) for q in spec.split(",")]
-        return dict(shots_per_lane_group=v[0], block=v[1], kernel_variant=v[2],
-                    xcd_remap=v[3] if v[3] > 1 else bool(v[3]),
-                    row_order=hilbert if len(v) > 4 and v[4] else None)
 
     if args.launch:
-        best = pinned(args.launch)
+        best = pinned_launch(args.launch, *coords)
     else:
         best, _ = autotune.tune(op, md, case["src"], case["rec"], precision=prec, steps=30, log=log, coords=coords,
                                 **base)
     if args.launch64:
-        best64 = pinned(args.launch64)
+        best64 = pinned_launch(args.launch64, *coords)
     else:
         best64, _ = autotune.tune(op, md, case["src"][:1], case["rec"], precision=gpu.FP64, steps=30, log=log,
                                   coords=coords, **base)
     log(f"tuned batched {autotune.describe(best)}; tuned fp64 single {autotune.describe(best64)}")
-    kw = dict(base, **best)
-    kw64 = dict(base, **best64)
+    kw = dict(base, **autotune.run_kwargs(best))
+    kw64 = dict(base, **autotune.run_kwargs(best64))
 
     # the step kernel with every shot active (untimed for the figure): 2 x 20 launches
     warm = gpu.Run(op, md, case["src"], 40, case["rec"], precision=prec, **dict(kw, guard_every=20))
@@ -331,9 +344,10 @@ def main():
             mmd = gpu.Medium(mop, p["c"], p["mass_lump"], p["dirichlet"], p["mpml"])
             keep += [mop, mmd]
             morder = None if kw.get("row_order") is None else host.locality_order(p["x"], p["y"])
+            # single-shot models in the sliced layout (one coalesced load per matrix and slot)
             runs.append(gpu.Run(mop, mmd, c["src"], L, c["rec"], c["dt"], c["f0"], c["t0"], precision=prec,
                                 nsteps=c["nsteps"], block=kw.get("block", 0), shots_per_lane_group=1,
-                                kernel_variant=kw.get("kernel_variant", 0), row_order=morder))
+                                kernel_variant=8, row_order=morder))
         grp = gpu.Multi(runs)
         grp.advance(L)
         ms_m = grp.last_ms_per_step

@@ -151,14 +151,19 @@ typedef struct {
                                   (6: 2 per trip on rows with an M-PML profile in the FP32 forms),
                                   7 = LDS-staged gathers: each
                                   block copies the fields of its rows' neighbour nodes into LDS once
-                                  (16, 32 or 64 shots per lane group only).  Same results (bit for
+                                  (16, 32 or 64 shots per lane group only), 8 = sliced layout for one
+                                  shot per lane group: the matrix is stored so that nonzero k of the
+                                  64 rows of a wave is one coalesced load per matrix (rows sorted by
+                                  length within windows of sell_sigma rows of the row order).  Same results (bit for
                                   bit in FP64; in the FP32 forms variant 7 can differ in the sign
                                   of a zero); a multi-run launch runs 4 to 6 as 5 in the FP32 forms
                                   and as 4 in FP64, 3 as 3 for 32- and 64-shot groups (else as
-                                  1), 7 as 1 */
+                                  1), 7 as 1, 8 as 8 */
     const int *row_order;      /* [node_num] permutation: the order in which the step kernel's
                                   waves take rows (e.g. sfh_locality_order), NULL = node order.
                                   Changes only which rows run together, not any result */
+    int sell_sigma;            /* kernel_variant 8: rows per sorting window, a multiple of 64
+                                  (64 = no sorting), 0 = 128 */
 } sfg_shot_desc;
 
 /* Creates a run with all state zeroed (elastic_wave.c:466-502). */

@@ -27,7 +27,7 @@ class ShotDesc(C.Structure):
         ("dt", _d), ("f0", _d), ("t0", _d), ("amplitude", _d), ("angle_deg", _d),
         ("block", _i), ("shots_per_lane_group", _i), ("guard_every", _i), ("use_graph", _i),
         ("xcd_remap", _i), ("src_count", _pi), ("src_nodes", _pi), ("src_weight", _pd),
-        ("kernel_variant", _i), ("row_order", _pi),
+        ("kernel_variant", _i), ("row_order", _pi), ("sell_sigma", _i),
     ]
 
 

@@ -1,15 +1,15 @@
 """Autotuner for the step kernel: picks the block size, the lane grouping
-(rows per wavefront vs shots per lane group), the kernel variant, the XCD
-block map (contiguous ranges, round-robin, or stripes of Q row blocks) and the
-row order (node order or a Hilbert walk over the node coordinates) by timing
-short runs of the actual problem on the GPU.
+(rows per wavefront vs shots per lane group), the kernel variant (with the
+sliced layouts' sorting window), the XCD block map (contiguous ranges,
+round-robin, or stripes of Q row blocks) and the row order (node order, a
+Hilbert walk over the node coordinates, or patches of a structured mesh) by
+timing short runs of the actual problem on the GPU.
 
 Each candidate runs ``steps`` steps (after ``steps`` untimed ones) as one
 captured graph with the energy guard at the end, timed with HIP events on the
 run's stream; the fastest candidate wins.  Results are cached per (nodes, nnz,
 shots, precision).  Every candidate gives the same results bit for bit.
 """
-import itertools
 import math
 
 from . import gpu, host
@@ -25,11 +25,25 @@ def stripe_widths(nodes, sp, block):
     return sorted({max(1, int(round(q * f))) for f in (0.5, 1.0, 2.0)})
 
 
-def candidates(nshots, nodes=160801):
-    """(shots per lane group, block, kernel variant, xcd remap, row order) to try."""
+def candidates(nshots, nodes=160801, patches=False):
+    """Launch shapes to try, as gpu.Run keyword dicts (row_order by name: "node",
+    "hilbert", or "patchWxH" for patch_order(W, H))."""
+    def c(sp, block, variant, xcd, order, sigma=0):
+        return dict(shots_per_lane_group=sp, block=block, kernel_variant=variant, xcd_remap=xcd, row_order=order,
+                    sell_sigma=sigma)
+
     if nshots <= 1:
-        sps, blocks = [1], [64, 128]
-    elif nshots <= 4:
+        # one row per lane: per-row layouts, the sliced layout (8) and, on meshes
+        # with a line structure, the sliced layout with LDS-staged halos over
+        # patches of the mesh (9)
+        cands = [c(1, b, v, x, o) for b in (64, 128) for v in (1, 5) for x in (True, False)
+                 for o in ("node", "hilbert")]
+        cands += [c(1, b, 8, x, "node", sg) for b in (128, 256) for x in (True, False) for sg in (0, 64)]
+        if patches:
+            cands += [c(1, b, 9, x, o, sg) for b, o in ((256, "patch16x16"), (256, "patch32x8"), (64, "patch16x4"))
+                      for x in (True, False) for sg in (0, 64)]
+        return cands
+    if nshots <= 4:
         sps, blocks = [4, 16], [256]
     elif nshots <= 16:
         sps, blocks = [16, 64], [256]
@@ -39,31 +53,36 @@ def candidates(nshots, nodes=160801):
     # measured about 2x slower than 5 on Q4 400x400 x 128 shots (the block's
     # staging and its compute do not overlap at 1-3 blocks per CU)
     variants = [1, 3, 4, 5, 6]
-    cands = list(itertools.product(sps, blocks, variants, [True, False], ["node", "hilbert"]))
-    if nshots > 1:
-        # XCD stripes of Q row blocks (node order): all XCDs sweep the rows together;
-        # Q near (row blocks per grid line) / 8 keeps vertical neighbours in one L2
-        for sp, block in itertools.product(sps, blocks):
-            qs = [q for q in stripe_widths(nodes, sp, block) if q > 1]
-            cands += list(itertools.product([sp], [block], variants, qs, ["node"]))
+    cands = [c(sp, b, v, x, o) for sp in sps for b in blocks for v in variants for x in (True, False)
+             for o in ("node", "hilbert")]
+    # XCD stripes of Q row blocks (node order): all XCDs sweep the rows together;
+    # Q near (row blocks per grid line) / 8 keeps vertical neighbours in one L2
+    for sp in sps:
+        for b in blocks:
+            cands += [c(sp, b, v, q, "node") for q in stripe_widths(nodes, sp, b) if q > 1 for v in variants]
     return cands
 
 
 def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None, coords=None):
     """Returns (best dict of gpu.Run keyword arguments, table of (candidate, ms/step)).
-    coords = (x, y) of the nodes enables the Hilbert row order candidates."""
+    coords = (x, y) of the nodes enables the Hilbert and patch row orders."""
     nshots = len(src)
     key = (op.node_num, op.nnz, nshots, precision, coords is not None)
     if key in _cache:
         return _cache[key]
     orders = {"node": None}
+    patches = False
     if coords is not None:
         orders["hilbert"] = host.locality_order(*coords)
-    def timed(sp, block, variant, xcd, oname, k):
+        if host.mesh_line_length(*coords) > 0:
+            patches = True
+            for w, h in ((16, 16), (32, 8), (16, 4)):
+                orders[f"patch{w}x{h}"] = host.patch_order(coords[0], coords[1], w, h)
+
+    def timed(cand, k):
         # two graph replays of k launches: the first warms up, the second is timed
-        run = gpu.Run(op, md, src, 2 * k, rec, dt, f0, t0, precision=precision, block=block,
-                      shots_per_lane_group=sp, kernel_variant=variant, xcd_remap=xcd, guard_every=k,
-                      row_order=orders[oname])
+        kw = dict(cand, row_order=orders[cand["row_order"]])
+        run = gpu.Run(op, md, src, 2 * k, rec, dt, f0, t0, precision=precision, guard_every=k, **kw)
         run.advance(k)
         run.advance(k)
         ms = run.last_ms_per_step
@@ -71,18 +90,17 @@ def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None
         return ms
 
     table = []
-    for sp, block, variant, xcd, oname in (cands or candidates(nshots, op.node_num)):
-        if oname not in orders:
+    for cand in (cands or candidates(nshots, op.node_num, patches)):
+        if cand["row_order"] not in orders:
             continue
         try:
-            ms = timed(sp, block, variant, xcd, oname, steps)
+            ms = timed(cand, steps)
         except gpu.NativeError as e:
             if e.code != -1:  # only an argument the shape cannot take (e.g. a halo beyond LDS) is skipped
                 raise
             if log:
-                log(f"tune skip sp={sp} block={block} variant={variant} order={oname}: {e}")
+                log(f"tune skip {cand}: {e}")
             continue
-        cand = dict(shots_per_lane_group=sp, block=block, kernel_variant=variant, xcd_remap=xcd, row_order=oname)
         table.append((cand, ms))
         if log:
             log(f"tune {cand}: {ms:.4f} ms/step")
@@ -91,12 +109,12 @@ def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None
     table.sort(key=lambda c: c[1])
     for i in range(min(3, len(table))):
         c, ms = table[i]
-        ms2 = timed(c["shots_per_lane_group"], c["block"], c["kernel_variant"], c["xcd_remap"], c["row_order"],
-                    2 * steps)
+        ms2 = timed(c, 2 * steps)
         table[i] = (c, min(ms, ms2))
         if log:
             log(f"tune again {c}: {ms2:.4f} ms/step")
     best = dict(min(table, key=lambda c: c[1])[0])
+    best["row_order_name"] = best["row_order"]
     best["row_order"] = orders[best["row_order"]]
     _cache[key] = (best, table)
     return best, table
@@ -105,5 +123,13 @@ def tune(op, md, src, rec, dt, f0, t0, precision, steps=40, cands=None, log=None
 def describe(best):
     """JSON-friendly form of a tuned launch (the row order by name)."""
     d = dict(best)
-    d["row_order"] = "node" if d.get("row_order") is None else "hilbert"
+    name = d.pop("row_order_name", None)
+    d["row_order"] = name or ("node" if d.get("row_order") is None else "given")
+    return d
+
+
+def run_kwargs(best):
+    """gpu.Run keyword arguments of a tuned launch."""
+    d = dict(best)
+    d.pop("row_order_name", None)
     return d

@@ -397,6 +397,9 @@ struct sfg_run {
     int *shot_steps = nullptr;  // [S] executed steps per internal shot (0 for padding)
     int *order = nullptr;       // row_order on the device (or null)
     int *hptr = nullptr, *hnode = nullptr, *kslot = nullptr, *rslot = nullptr; // LDS halos
+    int *sell_row = nullptr, *sell_off = nullptr, *sell_ci = nullptr; // sliced layout (variants 8, 9)
+    int *sell_own = nullptr;    // variant 9: LDS slot of each lane's own row
+    void *sell_val = nullptr;
     size_t lds_bytes = 0;       // dynamic LDS of the LDS-staged kernel
     int *src_shot = nullptr;    // src = per-node entry pointers, then shot and weights per entry
     double *src_wt = nullptr;
@@ -430,6 +433,11 @@ static void run_free(sfg_run *r)
     dfree(r->hnode);
     dfree(r->kslot);
     dfree(r->rslot);
+    dfree(r->sell_row);
+    dfree(r->sell_off);
+    dfree(r->sell_ci);
+    dfree(r->sell_own);
+    dfree(r->sell_val);
     dfree(r->src);
     dfree(r->src_shot);
     dfree(r->src_wt);
@@ -508,7 +516,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (!out || !op || !media || nmedia <= 0 || !d) return fail(SFG_E_ARG, "run: bad arguments");
     *out = nullptr;
     if (d->precision < 0 || d->precision > 3) return fail(SFG_E_ARG, "run: unknown precision %d", d->precision);
-    if (d->kernel_variant < 0 || d->kernel_variant > 7)
+    if (d->kernel_variant < 0 || d->kernel_variant > 9)
         return fail(SFG_E_ARG, "run: unknown kernel_variant %d", d->kernel_variant);
     if (d->xcd_remap < -1 || d->xcd_remap > (1 << 20))
         return fail(SFG_E_ARG, "run: xcd_remap %d (0, -1 or a stripe of 1 .. 2^20 row blocks)", d->xcd_remap);
@@ -714,6 +722,122 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
             return fail(SFG_E_HIP, "run: LDS size %zu: %s", r->lds_bytes, hipGetErrorString(ea));
         }
     }
+    // sliced layout (kernel variants 8 and 9): the row order cut into windows,
+    // each window's rows sorted by decreasing length (stable), then slices of 64
+    // consecutive rows; a slice's slots run to its longest row.  Variant 9 also
+    // stages each block's halo (the distinct columns of its rows) in LDS: its
+    // windows lie inside blocks (default: one window per block), and the slots
+    // hold the columns' positions in the block's sorted halo list.
+    std::vector<int> srow, soff, sci, ssrc, sown;
+    if (d->kernel_variant == 8 || d->kernel_variant == 9) {
+        const bool staged = d->kernel_variant == 9;
+        const int R = r->block;                                  // rows per block (one per lane)
+        const int win = d->sell_sigma > 0 ? d->sell_sigma : staged ? R : 128;
+        if (sp != 1 || win % 64 || (staged && R % win)) {
+            run_free(r);
+            return fail(SFG_E_ARG, "run: kernel_variant %d (sliced layout) needs one shot per lane group and a "
+                        "sorting window of a multiple of 64 rows%s (got %d shots per group, window %d)",
+                        d->kernel_variant, staged ? " dividing the block" : "", sp, win);
+        }
+        const size_t nslices = (size_t)r->nbx * (R / 64);
+        std::vector<int> logical(n);
+        for (int i = 0; i < n; ++i) logical[i] = d->row_order ? d->row_order[i] : i;
+        auto len = [&](int i) { return op->hrp[i + 1] - op->hrp[i]; };
+        for (int w0 = 0; w0 < n; w0 += win)
+            std::stable_sort(logical.begin() + w0, logical.begin() + std::min(n, w0 + win),
+                             [&](int x, int y) { return len(x) > len(y); });
+        srow.assign(nslices * 64, -1);
+        std::copy(logical.begin(), logical.end(), srow.begin());
+        soff.assign(nslices + 1, 0);
+        for (size_t sl = 0; sl < nslices; ++sl) {
+            int m = 0;
+            for (int l = 0; l < 64; ++l)
+                if (srow[sl * 64 + l] >= 0) m = std::max(m, len(srow[sl * 64 + l]));
+            if ((long long)soff[sl] + m > 0x7fffffffLL / 64) {
+                run_free(r);
+                return fail(SFG_E_ARG, "run: sliced layout beyond 2^31 slots");
+            }
+            soff[sl + 1] = soff[sl] + m;
+        }
+        // halo of each block (variant 9): its rows' columns, sorted, and their slots
+        std::vector<int> slot_of;
+        size_t maxh = 0;
+        if (staged) {
+            hptr.assign(r->nbx + 1, 0);
+            std::vector<int> stamp(n, -1), blk;
+            slot_of.assign(n, 0);
+            sown.assign(nslices * 64, 0);
+            for (int b = 0; b < r->nbx; ++b) {
+                blk.clear();
+                for (int l = b * R; l < (b + 1) * R; ++l) {
+                    const int i = srow[l];
+                    if (i < 0) continue;
+                    for (int k = op->hrp[i]; k < op->hrp[i + 1]; ++k)
+                        if (stamp[op->hci[k]] != b) { stamp[op->hci[k]] = b; blk.push_back(op->hci[k]); }
+                    if (stamp[i] != b) { stamp[i] = b; blk.push_back(i); }
+                }
+                std::sort(blk.begin(), blk.end());
+                for (size_t h = 0; h < blk.size(); ++h) slot_of[blk[h]] = (int)h;
+                for (int l = b * R; l < (b + 1) * R; ++l)
+                    if (srow[l] >= 0) sown[l] = slot_of[srow[l]];
+                hnode.insert(hnode.end(), blk.begin(), blk.end());
+                hptr[b + 1] = (int)hnode.size();
+                maxh = std::max(maxh, blk.size());
+                // slots of this block's slices, while slot_of holds its halo
+                for (int sl = b * (R / 64); sl < (b + 1) * (R / 64); ++sl)
+                    for (int l = 0; l < 64; ++l) {
+                        const int row = srow[(size_t)sl * 64 + l];
+                        if (row < 0) continue;
+                        for (int k = 0; k < soff[sl + 1] - soff[sl]; ++k) {
+                            const size_t q = ((size_t)soff[sl] + k) * 64 + l;
+                            if (sci.size() <= q) {
+                                sci.resize(std::max<size_t>((size_t)soff[nslices] * 64, 64), 0);
+                                ssrc.resize(sci.size(), -1);
+                            }
+                            if (k < len(row)) {
+                                sci[q] = slot_of[op->hci[op->hrp[row] + k]];
+                                ssrc[q] = op->hrp[row] + k;
+                            } else {
+                                sci[q] = slot_of[row];  // the row itself, value 0
+                            }
+                        }
+                    }
+            }
+            int lim = 0;
+            if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, g_device) != hipSuccess)
+                lim = 65536;
+            lds_stride = (int)maxh;
+            r->lds_bytes = 20 * maxh * w;
+            if (r->lds_bytes > (size_t)lim) {
+                run_free(r);
+                return fail(SFG_E_ARG, "run: a block's halo of %zu nodes needs %zu bytes of LDS (limit %d); use a "
+                            "smaller block or a row order that keeps blocks compact (a patch order)", maxh,
+                            r->lds_bytes, lim);
+            }
+            hipError_t ea = allow_step_lds(r->prec, 1, r->cps, r->lds_bytes);
+            if (ea != hipSuccess) {
+                run_free(r);
+                return fail(SFG_E_HIP, "run: LDS size %zu: %s", r->lds_bytes, hipGetErrorString(ea));
+            }
+        }
+        const size_t slots = (size_t)soff[nslices] * 64;
+        sci.resize(std::max<size_t>(slots, 64), 0);
+        ssrc.resize(sci.size(), -1);
+        if (!staged)
+            for (size_t sl = 0; sl < nslices; ++sl)
+                for (int l = 0; l < 64; ++l) {
+                    const int row = srow[sl * 64 + l];
+                    for (int k = 0; k < soff[sl + 1] - soff[sl]; ++k) {
+                        const size_t q = ((size_t)soff[sl] + k) * 64 + l;
+                        if (row >= 0 && k < len(row)) {
+                            sci[q] = op->hci[op->hrp[row] + k];
+                            ssrc[q] = op->hrp[row] + k;
+                        } else {
+                            sci[q] = row >= 0 ? row : 0;   // column = the row itself, value 0
+                        }
+                    }
+                }
+    }
     int rc = SFG_OK;
     hipError_t e = hipSuccess;
     if ((rc = dalloc((char **)&r->state, 2 * NFIELD * F * w)) ||
@@ -751,8 +875,41 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         e = hipMemcpyAsync(r->order, d->row_order, sizeof(int) * n, hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess && r->hptr) e = hipMemcpyAsync(r->hptr, hptr.data(), sizeof(int) * hptr.size(), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess && r->hnode) e = hipMemcpyAsync(r->hnode, hnode.data(), sizeof(int) * hnode.size(), hipMemcpyHostToDevice, r->stream);
-    if (e == hipSuccess && r->kslot) e = hipMemcpyAsync(r->kslot, kslot.data(), sizeof(int) * kslot.size(), hipMemcpyHostToDevice, r->stream);
-    if (e == hipSuccess && r->rslot) e = hipMemcpyAsync(r->rslot, rslot.data(), sizeof(int) * rslot.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess && r->kslot && !kslot.empty())
+        e = hipMemcpyAsync(r->kslot, kslot.data(), sizeof(int) * kslot.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess && r->rslot && !rslot.empty())
+        e = hipMemcpyAsync(r->rslot, rslot.data(), sizeof(int) * rslot.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess && !srow.empty()) {
+        int *dsrc = nullptr;
+        if ((rc = dalloc(&r->sell_row, srow.size())) || (rc = dalloc(&r->sell_off, soff.size())) ||
+            (rc = dalloc(&r->sell_ci, sci.size())) || (rc = dalloc((char **)&r->sell_val, sci.size() * 7 * w)) ||
+            (rc = dalloc(&dsrc, ssrc.size()))) {
+            dfree(dsrc);
+            run_free(r);
+            return rc;
+        }
+        if (!sown.empty()) {
+            if ((rc = dalloc(&r->sell_own, sown.size()))) {
+                dfree(dsrc);
+                run_free(r);
+                return rc;
+            }
+            e = hipMemcpyAsync(r->sell_own, sown.data(), sizeof(int) * sown.size(), hipMemcpyHostToDevice, r->stream);
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(r->sell_row, srow.data(), sizeof(int) * srow.size(), hipMemcpyHostToDevice, r->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(r->sell_off, soff.data(), sizeof(int) * soff.size(), hipMemcpyHostToDevice, r->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(r->sell_ci, sci.data(), sizeof(int) * sci.size(), hipMemcpyHostToDevice, r->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dsrc, ssrc.data(), sizeof(int) * ssrc.size(), hipMemcpyHostToDevice, r->stream);
+        if (e == hipSuccess)
+            e = launch_sell_values(r->prec == 0 ? (const void *)op->v64 : (const void *)op->v32, (size_t)op->nnz, dsrc,
+                                   sci.size(), (int)w, r->sell_val, r->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+        dfree(dsrc);
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(r->src, sptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->src_shot, sshot.data(), sizeof(int) * sshot.size(), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r->src_wt, swt.data(), sizeof(double) * swt.size(), hipMemcpyHostToDevice, r->stream);
@@ -823,10 +980,14 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
         a.kslot = r->kslot;
         a.rslot = r->rslot;
         a.lds_stride = lds_stride;
+        a.sell_row = r->sell_row;
+        a.sell_off = r->sell_off;
+        a.sell_ci = r->sell_ci;
+        a.sell_own = r->sell_own;
         a.ci = op->ci;
         for (int m = 0; m < 7; ++m) a.A[m] = vals + (size_t)m * op->nnz;
         // descriptor variant -> kernel variant (see step_tile)
-        static const int kv[8] = {1, 1, 0, 2, 3, 4, 5, 6};
+        static const int kv[10] = {1, 1, 0, 2, 3, 4, 5, 6, 7, 8};
         a.variant = kv[d->kernel_variant];
         a.coef = r->cps ? (decltype(coef))r->coefp : coef;
         a.flag = r->cps ? r->flagp : md->flag;
@@ -860,10 +1021,10 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     };
     if (r->prec == 0)
         fill(r->a64, (double *)r->state, (double *)r->comp, op->v64, md->coef64, [](double v) { return v; }),
-            r->a64.Ap = op->p64;
+            r->a64.Ap = op->p64, r->a64.sell_val = (const double *)r->sell_val;
     else
         fill(r->a32, (float *)r->state, (float *)r->comp, op->v32, md->coef32, [](double v) { return (float)v; }),
-            r->a32.Ap = op->p32;
+            r->a32.Ap = op->p32, r->a32.sell_val = (const float *)r->sell_val;
 
     // captured graphs of guard_every steps followed by the energy guard: one
     // per parity of the first step (a second one only when guard_every is odd)

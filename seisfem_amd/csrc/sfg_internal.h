@@ -93,6 +93,22 @@ struct StepArgs {
     // the block holding its row, rslot[i] = slot of row i itself.
     const int *hptr, *hnode, *kslot, *rslot;
     int lds_stride;
+    // Sliced layout for one row per lane (kernel variant 7, SP = 1): wave w of
+    // block b steps slice s = b * waves_per_block + w, whose 64 lanes take rows
+    // sell_row[64 s + lane] (-1 = none).  The slice's nonzero slots are
+    // sell_off[s] .. sell_off[s+1]-1; slot k of a lane holds its row's k-th
+    // nonzero (column order), column sell_ci[64 k + lane] and the value of
+    // matrix m (M, S1..S6) at sell_val[(7 k + m) * 64 + lane]; the slots past a
+    // row's length hold column = the row itself and value 0 (a sum plus an
+    // exact zero is unchanged).  Rows are sorted by length within windows of
+    // the row order so that a slice's rows are of about the same length.
+    const int *sell_row, *sell_off, *sell_ci;
+    const T *sell_val;
+    // LDS-staged sliced layout (kernel variant 8): block b stages the 20
+    // gathered fields of its halo nodes hnode[hptr[b] .. hptr[b+1]) into LDS as
+    // [field][slot] with field stride lds_stride; sell_ci then holds LDS slots
+    // and sell_own[64 s + lane] the slot of the lane's own row.
+    const int *sell_own;
     const T *A[7];          // M, S1..S6 on the union pattern (one array each)
     const T *Ap;            // the same values packed per nonzero: [nnz][8] = M, S1..S6, 0
     int variant;            // 0: separate value arrays; 1: packed values, gather loop unrolled x2
@@ -158,6 +174,10 @@ hipError_t launch_to_f32(const double *src, float *dst, size_t n, hipStream_t st
 // (formed in FP64) or 0 on clamped / zero-mass nodes
 hipError_t launch_medium_f32(int n, const double *coef64, const int *flag, float *coef32, hipStream_t st);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t st);
+// Values of the sliced layout: out[(7 k + m) * 64 + lane] = vals[m * nnz + src[64 k + lane]] for
+// src >= 0, else 0, over nslots = 64 * (slots); elem = 4 (float) or 8 (double)
+hipError_t launch_sell_values(const void *vals, size_t nnz, const int *src, size_t nslots, int elem, void *out,
+                              hipStream_t st);
 // pml[i] = 2 when coefficient DX..DYYP of row i is nonzero for one of the ns
 // shots (coef is [NCOEF][n][ns]; ns = 1: one medium for every shot), 1 when
 // row i is a column of such a row in the pattern rp/ci, else 0

@@ -142,22 +142,34 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
     constexpr bool KAHAN = (PREC >= 2);
     constexpr bool C16 = (PREC == 3);           // Kahan terms stored as bfloat16 pairs
     constexpr bool LDSG = V == 6;
-    constexpr bool PACKED = V >= 1;
+    // 7: the sliced layout of one row per lane (StepArgs::sell_*): nonzero k of
+    // the 64 rows of a wave is one coalesced load per matrix; 8: the same with
+    // the 20 gathered fields of the block's halo nodes staged in LDS once
+    // (field-major state), the slices' columns holding LDS slots
+    constexpr bool LDS1 = V == 8;
+    constexpr bool SELL = V == 7 || V == 8;
+    constexpr bool LDSX = LDSG || LDS1;          // neighbours' fields come from LDS
+    static_assert(!SELL || SP == 1, "the sliced layout holds one row per lane");
+    constexpr bool PACKED = V >= 1 && !SELL;
+    // sliced walks: slots whose loads are in flight ahead of their use
+    constexpr int SELL_AHEAD = EXACT ? 3 : 4;
     constexpr bool NT = V >= 2;
     constexpr bool PRE = V >= 3;                 // own-row accelerations / Kahan terms first
-    constexpr bool CAP = V >= 3 && !LDSG;        // own-row fields from the diagonal gather
+    constexpr bool CAP = V >= 3 && !LDSX;        // own-row fields from the diagonal gather
     constexpr int UN = V == 5 ? 4 : V == 4 ? 3 : 2;  // nonzeros per trip of the gather loop
     // the full 20-field walk of M-PML rows in variant 5 keeps 2 per trip: its
     // registers, not the short walk's, set the kernel's occupancy
     constexpr int UNP = (V == 5 && !EXACT) ? 2 : UN;
     constexpr bool WUW = !EXACT;                 // store U and W of the new state
-    constexpr bool SKIP = !EXACT && !LDSG;       // interior rows gather 10 values per neighbour
+    constexpr bool SKIP = !EXACT && !LDSX;       // interior rows gather 10 values per neighbour
     constexpr int RPW = (SP >= 64) ? 1 : 64 / SP;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
     int row = (bx * wpb + wave) * RPW + (SP >= 64 ? 0 : lane / SP);
     if constexpr (SP >= 64) row = __builtin_amdgcn_readfirstlane(row);
+    const int slice = bx * wpb + wave;           // sliced layout: the wave's slice of 64 rows
+    if constexpr (SELL) row = ro(a.sell_row)[(size_t)slice * 64 + lane];
     constexpr int GS = SP >= 64 ? 64 : SP;      // shots of one lane group
     const int ls = SP >= 64 ? lane : lane % SP;  // this lane's shot within the group
     const int s = by * GS + ls;
@@ -199,8 +211,30 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         }
         __syncthreads();
     }
-    if (row >= a.n) return;
-    if (a.order) {
+    if constexpr (LDS1) {
+        // a finished shot skips the whole block (uniform, before the barrier)
+        if (by >= nact) return;
+        // each thread copies the 20 gathered fields of halo nodes h, h + blockDim.x, ...
+        // (20 independent loads in flight; consecutive threads read consecutive halo
+        // nodes, which are sorted) to LDS [field][slot]
+        const T *inb = a.state + (size_t)(t & 1) * NFIELD * a.fstride;
+        const int h0 = ro(a.hptr)[bx], nh = ro(a.hptr)[bx + 1] - h0;
+        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+            const size_t node = (size_t)a.hnode[h0 + h] * a.S + by;
+            T buf[20];
+#pragma unroll
+            for (int f20 = 0; f20 < 20; ++f20) buf[f20] = inb[(size_t)(f20 < 12 ? f20 : f20 + 6) * a.fstride + node];
+#pragma unroll
+            for (int f20 = 0; f20 < 20; ++f20) lds[f20 * a.lds_stride + h] = buf[f20];
+        }
+        __syncthreads();
+    }
+    if constexpr (SELL) {
+        if (row < 0) return;                     // a lane past the slices' last row
+    } else if (row >= a.n) {
+        return;
+    }
+    if (!SELL && a.order) {
         // rows in locality order: the waves in flight hold a compact patch of the mesh
         row = ro(a.order)[row];
         if constexpr (SP >= 64) row = __builtin_amdgcn_readfirstlane(row);
@@ -288,18 +322,32 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         T s4u = 0, s4w = 0, s5u = 0, s5w = 0, s6u = 0, s6w = 0;
         T mu1t = 0, mu1 = 0, mlx1 = 0, mlx2 = 0, mu2t = 0, mu2 = 0, mu3t = 0, mu3 = 0, mlx3 = 0, mlx4 = 0;
         T mw1t = 0, mw1 = 0, mly1 = 0, mly2 = 0, mw2t = 0, mw2 = 0, mw3t = 0, mw3 = 0, mly3 = 0, mly4 = 0;
-        const int k0 = ro(a.rp)[row], k1 = ro(a.rp)[row + 1];
+        // nonzeros k0 .. k1-1 of the row, or slots k0 .. k1-1 of its slice
+        const int k0 = SELL ? ro(a.sell_off)[slice] : ro(a.rp)[row];
+        const int k1 = SELL ? ro(a.sell_off)[slice + 1] : ro(a.rp)[row + 1];
         const size_t o = node_base(row);
         T cur[NFIELD];
         T own_u = 0, own_w = 0;                  // summed rows: the row's own U, W
         bool got = false;
-        auto nonzero = [&](int k) __attribute__((always_inline)) {
-            const int j = LDSG ? 0 : ro(a.ci)[k];
+        // one slot of the sliced layout: column (or LDS slot) and the 7 values
+        struct Slot {
+            int j;
+            T v[7];
+        };
+        // nonzero k; the sliced walk hands over its slot already loaded
+        auto nonzero = [&](int k, auto... pre) __attribute__((always_inline)) {
+            int j;
             T am, a1, a2, a3, a4, a5, a6;
-            if constexpr (PACKED) {
+            if constexpr (sizeof...(pre) > 0) {
+                const Slot &q = (pre, ...);
+                j = q.j;
+                am = q.v[0]; a1 = q.v[1]; a2 = q.v[2]; a3 = q.v[3]; a4 = q.v[4]; a5 = q.v[5]; a6 = q.v[6];
+            } else if constexpr (PACKED) {
+                j = LDSG ? 0 : ro(a.ci)[k];
                 const auto v = ro(a.Ap) + (size_t)k * 8;
                 am = v[0]; a1 = v[1]; a2 = v[2]; a3 = v[3]; a4 = v[4]; a5 = v[5]; a6 = v[6];
             } else {
+                j = LDSG ? 0 : ro(a.ci)[k];
                 am = ro(a.A[0])[k];
                 a1 = ro(a.A[1])[k]; a2 = ro(a.A[2])[k]; a3 = ro(a.A[3])[k];
                 a4 = ro(a.A[4])[k]; a5 = ro(a.A[5])[k]; a6 = ro(a.A[6])[k];
@@ -310,12 +358,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             if constexpr (LDSG) {
                 x = lds + ro(a.kslot)[k] * a.lds_stride + ls;
                 xs = GS;
+            } else if constexpr (LDS1) {
+                x = lds + j;                     // j is the neighbour's LDS slot
+                xs = a.lds_stride;
             } else {
                 x = in + node_base(j);
                 xs = FS;
             }
             auto fld = [&](int f) __attribute__((always_inline)) {
-                return LDSG ? x[gathered_slot(f) * xs] : x[f * xs];
+                return LDSX ? x[gathered_slot(f) * xs] : x[f * xs];
             };
             const T u1 = fld(U1), u2 = fld(U2), u3 = fld(U3);
             const T w1 = fld(W1), w2 = fld(W2), w3 = fld(W3);
@@ -354,13 +405,19 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             mly3 = mac<EXACT>(mly3, am, ly3); mly4 = mac<EXACT>(mly4, am, ly4);
         };
         // interior row: U, W and the memory fields of neighbour j
-        auto nonzero_in = [&](int k) __attribute__((always_inline)) {
-            const int j = ro(a.ci)[k];
+        auto nonzero_in = [&](int k, auto... pre) __attribute__((always_inline)) {
+            int j;
             T am, a1, a2, a3, a4;
-            if constexpr (PACKED) {
+            if constexpr (sizeof...(pre) > 0) {
+                const Slot &q = (pre, ...);
+                j = q.j;
+                am = q.v[0]; a1 = q.v[1]; a2 = q.v[2]; a3 = q.v[3]; a4 = q.v[4];
+            } else if constexpr (PACKED) {
+                j = ro(a.ci)[k];
                 const auto v = ro(a.Ap) + (size_t)k * 8;
                 am = v[0]; a1 = v[1]; a2 = v[2]; a3 = v[3]; a4 = v[4];
             } else {
+                j = ro(a.ci)[k];
                 am = ro(a.A[0])[k];
                 a1 = ro(a.A[1])[k]; a2 = ro(a.A[2])[k]; a3 = ro(a.A[3])[k]; a4 = ro(a.A[4])[k];
             }
@@ -387,9 +444,38 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             mly1 = mac<EXACT>(mly1, am, ly1); mly2 = mac<EXACT>(mly2, am, ly2);
             mly3 = mac<EXACT>(mly3, am, ly3); mly4 = mac<EXACT>(mly4, am, ly4);
         };
-        auto walk = [&](auto &&nz, auto per_trip) __attribute__((always_inline)) {
+        // sliced layout: slot k's column and its first NV values (the rest 0)
+        auto load_slot = [&](int k, auto nv) __attribute__((always_inline)) {
+            constexpr int NV = decltype(nv)::value;
+            Slot q;
+            q.j = ld_once<true>(a.sell_ci + (size_t)k * 64 + lane);
+            const T *v = a.sell_val + (size_t)k * (7 * 64) + lane;
+#pragma unroll
+            for (int m = 0; m < 7; ++m) q.v[m] = m < NV ? ld_once<true>(v + 64 * m) : T(0);
+            return q;
+        };
+        auto walk = [&](auto &&nz, auto per_trip, auto nv) __attribute__((always_inline)) {
             constexpr int U = decltype(per_trip)::value;
-            if constexpr (PACKED) {
+            if constexpr (SELL) {
+                // the slots' loads run SELL_AHEAD slots ahead of their use (a ring of
+                // registers), so a wave keeps several slots' loads in flight
+                constexpr int PA = SELL_AHEAD;
+                Slot ring[PA];
+#pragma unroll
+                for (int p = 0; p < PA; ++p)
+                    if (k0 + p < k1) ring[p] = load_slot(k0 + p, nv);
+                for (int kb = k0; kb < k1; kb += PA) {
+#pragma unroll
+                    for (int p = 0; p < PA; ++p) {
+                        const int k = kb + p;
+                        if (k < k1) {
+                            const Slot q = ring[p];
+                            if (k + PA < k1) ring[p] = load_slot(k + PA, nv);
+                            nz(k, q);
+                        }
+                    }
+                }
+            } else if constexpr (PACKED) {
                 // U nonzeros per trip: their gathers are in flight together; each
                 // sum still runs in column order
                 int k = k0;
@@ -410,9 +496,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                     for (int f = U1; f <= W3; ++f) cur[f] = ld_once<NT>(in + f * FS + o);
                 }
             }
-            walk(nonzero_in, std::integral_constant<int, UN>{});
+            walk(nonzero_in, std::integral_constant<int, UN>{}, std::integral_constant<int, 5>{});
         } else {
-            walk(nonzero, std::integral_constant<int, UNP>{});
+            walk(nonzero, std::integral_constant<int, UNP>{}, std::integral_constant<int, 7>{});
         }
 
         // ---- per-node coefficients --------------------------------------
@@ -469,6 +555,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
 #pragma unroll
             for (int f = 0; f < NFIELD; ++f)
                 if (f < U1TT || f > W3TT) cur[f] = x[gathered_slot(f) * GS];
+        } else if constexpr (LDS1) {
+            const T *x = lds + a.sell_own[(size_t)slice * 64 + lane];
+#pragma unroll
+            for (int f = 0; f < NFIELD; ++f)
+                if (f < U1TT || f > W3TT) cur[f] = x[gathered_slot(f) * a.lds_stride];
         } else if (interior) {
             if constexpr (!PRE) {
                 if (!agg) {
@@ -671,7 +762,7 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
     // (FP64 with 3 per trip: 256 VGPRs, one wave).
     constexpr int D = PREC == 0 ? 3 : 4;
     const int v = a.variant;
-    const int w = v == 0 ? 0 : (v >= 3 && v <= 5) ? 3 : v == 2 ? 2 : 1;
+    const int w = v == 0 ? 0 : (v >= 3 && v <= 5) ? 3 : v == 2 ? 2 : 1;   // 7 (sliced) only with sp == 1
 #define SFG_T(SPV, C, VV) step_tile<T, PREC, SPV, C, VV>(a, t, bx, by)
 #define SFG_G(SPV, C)                                  \
     do {                                               \
@@ -681,8 +772,9 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
         else SFG_T(SPV, C, 0);                         \
     } while (0)
     if (a.sp == 1) {
-        if (a.cps) { if (w) SFG_T(1, true, 1); else SFG_T(1, true, 0); }
-        else { if (w) SFG_T(1, false, 1); else SFG_T(1, false, 0); }
+        // single-shot runs in the sliced layout (7) or the per-row layouts
+        if (a.cps) { if (v == 7) SFG_T(1, true, 7); else if (w) SFG_T(1, true, 1); else SFG_T(1, true, 0); }
+        else { if (v == 7) SFG_T(1, false, 7); else if (w) SFG_T(1, false, 1); else SFG_T(1, false, 0); }
     } else if (a.sp == 32) {
         if (a.cps) SFG_G(32, true);
         else SFG_G(32, false);
@@ -760,6 +852,18 @@ static hipError_t launch_step_t(int sp, bool cps, const void *args, dim3 g, int 
     case 4: return launch_step_v<T, PREC, 4>(sp, cps, a, g, block, slot, 0, st);
     case 5: return launch_step_v<T, PREC, 5>(sp, cps, a, g, block, slot, 0, st);
     case 6: return launch_step_lds<T, PREC>(sp, cps, a, g, block, slot, lds, st);
+    case 7:
+        // sliced layout: one row per lane only
+        if (sp != 1) return hipErrorInvalidValue;
+        if (cps) hipLaunchKernelGGL((step_kernel<T, PREC, 1, true, 7>), g, dim3(block), 0, st, a, slot);
+        else hipLaunchKernelGGL((step_kernel<T, PREC, 1, false, 7>), g, dim3(block), 0, st, a, slot);
+        return hipGetLastError();
+    case 8:
+        // sliced layout with the halo staged in LDS: one row per lane only
+        if (sp != 1) return hipErrorInvalidValue;
+        if (cps) hipLaunchKernelGGL((step_kernel<T, PREC, 1, true, 8>), g, dim3(block), lds, st, a, slot);
+        else hipLaunchKernelGGL((step_kernel<T, PREC, 1, false, 8>), g, dim3(block), lds, st, a, slot);
+        return hipGetLastError();
     default: return hipErrorInvalidValue;
     }
 }
@@ -771,6 +875,7 @@ static hipError_t allow_lds_t(int sp, bool cps, int bytes)
 {
     const void *f = nullptr;
     switch (sp) {
+    case 1: f = cps ? (const void *)step_kernel<T, PREC, 1, true, 8> : (const void *)step_kernel<T, PREC, 1, false, 8>; break;
     case 16: f = cps ? (const void *)step_kernel<T, PREC, 16, true, 6> : (const void *)step_kernel<T, PREC, 16, false, 6>; break;
     case 32: f = cps ? (const void *)step_kernel<T, PREC, 32, true, 6> : (const void *)step_kernel<T, PREC, 32, false, 6>; break;
     case 64: f = cps ? (const void *)step_kernel<T, PREC, 64, true, 6> : (const void *)step_kernel<T, PREC, 64, false, 6>; break;
@@ -1115,6 +1220,32 @@ hipError_t launch_pack_per_shot(const void *const *srcs_dev, int nsrc, int ncoef
     else if (elem == 4)
         hipLaunchKernelGGL(pack_per_shot_kernel<uint32_t>, dim3(2048), dim3(256), 0, st,
                            (const uint32_t *const *)srcs_dev, nsrc, ncoef, n, S, (uint32_t *)dst);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <class E>
+__global__ void sell_values_kernel(const E *vals, size_t nnz, const int *src, size_t nslots, E *out)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < nslots; q += (size_t)gridDim.x * blockDim.x) {
+        const int k = src[q];
+        const size_t base = (q / 64) * (7 * 64) + q % 64;
+#pragma unroll
+        for (int m = 0; m < 7; ++m) out[base + (size_t)m * 64] = k >= 0 ? vals[(size_t)m * nnz + k] : E(0);
+    }
+}
+
+hipError_t launch_sell_values(const void *vals, size_t nnz, const int *src, size_t nslots, int elem, void *out,
+                              hipStream_t st)
+{
+    if (nslots % 64) return hipErrorInvalidValue;
+    if (elem == 8)
+        hipLaunchKernelGGL(sell_values_kernel<double>, dim3(2048), dim3(256), 0, st, (const double *)vals, nnz, src,
+                           nslots, (double *)out);
+    else if (elem == 4)
+        hipLaunchKernelGGL(sell_values_kernel<float>, dim3(2048), dim3(256), 0, st, (const float *)vals, nnz, src,
+                           nslots, (float *)out);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -88,7 +88,8 @@ class Medium:
 class Run:
     def __init__(self, op, md, src_nodes, max_steps, rec_nodes, dt, f0, t0, amplitude=1.0e10,
                  angle_deg=90.0, precision=FP64, nsteps=None, block=0, shots_per_lane_group=0,
-                 guard_every=0, use_graph=True, xcd_remap=True, sources=None, kernel_variant=0, row_order=None):
+                 guard_every=0, use_graph=True, xcd_remap=True, sources=None, kernel_variant=0, row_order=None,
+                 sell_sigma=0):
         """sources: optional per-shot list of (nodes, weights[k][2]) distributed
         sources (u and w weights on each node) replacing the point forces.
         row_order: optional permutation of the nodes giving the order in which
@@ -114,6 +115,7 @@ class Run:
         # True: contiguous ranges per XCD; False: round-robin; an int Q > 1: stripes of Q row blocks
         d.xcd_remap = xcd_remap if (type(xcd_remap) is int and xcd_remap > 1) else (0 if xcd_remap else -1)
         d.kernel_variant = int(kernel_variant)
+        d.sell_sigma = int(sell_sigma)
         if row_order is not None:
             self._order = np.ascontiguousarray(row_order, np.int32)
             d.row_order = _pi(self._order)

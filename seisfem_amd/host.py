@@ -111,6 +111,34 @@ def locality_order(x, y):
     return order
 
 
+def mesh_line_length(x, y):
+    """Nodes per grid line of a structured mesh numbered line by line (the
+    reference's generators, mesh/mesh_xy_*.c): the first node at which the
+    coordinate that grows along a line falls back.  0 if there is none."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    for c in (x, y):
+        drop = np.flatnonzero(np.diff(c) < 0)
+        if drop.size:
+            return int(drop[0] + 1)
+    return 0
+
+
+def patch_order(x, y, pw=16, ph=16):
+    """Rows in patches of pw nodes x ph grid lines of a structured mesh,
+    patches line by line, each patch's rows line by line: a block of pw*ph
+    rows then covers a compact piece of the mesh whose halo is small (the
+    LDS-staged kernel variant), while every pw rows stay consecutive nodes.
+    Node order when the mesh has no line structure."""
+    n = len(x)
+    line = mesh_line_length(x, y)
+    ids = np.arange(n, dtype=np.int64)
+    if line <= 0:
+        return ids.astype(np.int32)
+    ix, iy = ids % line, ids // line
+    return np.lexsort((ix, iy, ix // pw, iy // ph)).astype(np.int32)
+
+
 def dirichlet_flags(x, y, edge_size, xmin, xmax, ymin, ymax, free_surface):
     f = np.empty(x.size, np.int32)
     lib().sfh_dirichlet_flags(x.size, _pd(x), _pd(y), edge_size, xmin, xmax, ymin, ymax,

@@ -87,6 +87,141 @@ def test_fp64_loop_bit_identical(tc, nelem, steps, graph, guard, xcd, variant):
     assert bits_equal(res["seis_w"][0], ref["seis_w"])
 
 
+@pytest.mark.parametrize("tc,nelem,steps,sigma,block,hilbert", [
+    (host.Q4, 20, 150, 0, 64, False), (host.Q9, 10, 120, 0, 128, False), (host.Q9, 10, 120, 64, 64, False),
+    (host.T6, 10, 120, 0, 256, True), (host.T3, 20, 120, 256, 64, False), (host.Q9, 9, 80, 192, 192, True)])
+def test_sliced_layout_fp64_bit_identical(tc, nelem, steps, sigma, block, hilbert):
+    """Kernel variant 8 (one row per lane, matrix in slices of 64 rows sorted
+    by length within windows of the row order): bit-identical to the oracle,
+    with and without sorting, in node and Hilbert row order."""
+    case = _case(tc, nelem, steps)
+    prob = case["problem"]
+    rec = np.arange(prob["node_num"], dtype=np.int32)
+    order = host.locality_order(prob["x"], prob["y"]) if hilbert else None
+    say(f"sliced layout FP64 {host.TYPE_NAMES[tc]} {nelem}x{nelem} steps={steps} window={sigma} block={block} "
+        f"hilbert={hilbert}")
+    res = gpu.simulate(prob, case["src"], steps, rec, case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
+                       kernel_variant=8, sell_sigma=sigma, block=block, row_order=order)
+    ref = oracle.oracle_masslump(prob, case["src"][0], steps + 2, rec, case["dt"], case["f0"], case["t0"])
+    assert ref["status"] == steps and np.max(np.abs(ref["seis_u"])) > 0
+    assert bits_equal(res["seis_u"][0], ref["seis_u"])
+    assert bits_equal(res["seis_w"][0], ref["seis_w"])
+
+
+@pytest.mark.parametrize("tc,nelem,steps,block,order", [
+    (host.Q4, 20, 150, 256, "patch"), (host.Q9, 10, 120, 256, "patch"), (host.Q9, 11, 100, 128, "node"),
+    (host.T6, 10, 120, 256, "patch"), (host.T3, 20, 120, 64, "hilbert"), (host.Q4, 21, 90, 192, "patch")])
+def test_lds_staged_sliced_layout_fp64_bit_identical(tc, nelem, steps, block, order):
+    """Kernel variant 9 (sliced layout, each block's halo nodes staged in LDS
+    once): bit-identical to the oracle in patch, node and Hilbert row orders."""
+    case = _case(tc, nelem, steps)
+    prob = case["problem"]
+    x, y = prob["x"], prob["y"]
+    rec = np.arange(prob["node_num"], dtype=np.int32)
+    ro = {"patch": lambda: host.patch_order(x, y, 16, block // 16), "node": lambda: None,
+          "hilbert": lambda: host.locality_order(x, y)}[order]()
+    say(f"LDS-staged sliced layout FP64 {host.TYPE_NAMES[tc]} {nelem}x{nelem} steps={steps} block={block} {order}")
+    res = gpu.simulate(prob, case["src"], steps, rec, case["dt"], case["f0"], case["t0"], precision=gpu.FP64,
+                       kernel_variant=9, block=block, row_order=ro)
+    ref = oracle.oracle_masslump(prob, case["src"][0], steps + 2, rec, case["dt"], case["f0"], case["t0"])
+    assert ref["status"] == steps and np.max(np.abs(ref["seis_u"])) > 0
+    assert bits_equal(res["seis_u"][0], ref["seis_u"])
+    assert bits_equal(res["seis_w"][0], ref["seis_w"])
+
+
+@pytest.mark.parametrize("prec", [gpu.FP32, gpu.FP32_KAHAN, gpu.FP32_KAHAN16])
+def test_lds_staged_sliced_layout_fp32_forms_match_row_layout(prec):
+    """The FP32 forms of variant 9 walk every neighbour's 20 fields from LDS
+    (rows outside the damping layers too, whose extra products are exact
+    zeros): within 1e-6 rel-L2 of the per-row layout, every node a receiver,
+    three shots of one-row-per-lane groups, patch order."""
+    case = _case(host.Q9, 12, 100, nshots=3)
+    prob = case["problem"]
+    n = prob["node_num"]
+    rec = np.arange(n, dtype=np.int32)
+    say(f"LDS-staged sliced layout precision={prec}: Q9 12x12, 3 shots, vs kernel variant 1")
+    op = gpu.Operator(n, prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    out = []
+    for kw in (dict(kernel_variant=1), dict(kernel_variant=9, row_order=host.patch_order(prob["x"], prob["y"], 16, 8))):
+        run = gpu.Run(op, md, case["src"], 100, rec, case["dt"], case["f0"], case["t0"], precision=prec,
+                      shots_per_lane_group=1, block=128, **kw)
+        run.advance(100)
+        out.append(run.seismograms())
+        run.close()
+    (u1, w1), (u9, w9) = out
+    a = np.concatenate([u9.ravel(), w9.ravel()])
+    b = np.concatenate([u1.ravel(), w1.ravel()])
+    assert np.max(np.abs(b)) > 0 and rel_l2(a, b) < 1e-6
+    md.close()
+    op.close()
+
+
+@pytest.mark.parametrize("prec", [gpu.FP32, gpu.FP32_KAHAN, gpu.FP32_KAHAN16])
+def test_sliced_layout_fp32_forms_equal_row_layout(prec):
+    """The FP32 forms in the sliced layout add the same products in the same
+    order as the per-row layout (the padded slots add exact zeros): equal
+    seismograms at every node, three shots of one-row-per-lane groups."""
+    case = _case(host.Q9, 12, 100, nshots=3)
+    prob = case["problem"]
+    n = prob["node_num"]
+    rec = np.arange(n, dtype=np.int32)
+    say(f"sliced layout precision={prec}: Q9 12x12, 3 shots, vs kernel variant 1")
+    op = gpu.Operator(n, prob["csr"])
+    md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+    out = []
+    for variant in (1, 8):
+        run = gpu.Run(op, md, case["src"], 100, rec, case["dt"], case["f0"], case["t0"], precision=prec,
+                      shots_per_lane_group=1, kernel_variant=variant, block=128)
+        run.advance(100)
+        out.append(run.seismograms())
+        run.close()
+    (u1, w1), (u8, w8) = out
+    assert np.max(np.abs(u1)) > 0
+    assert np.array_equal(u1, u8) and np.array_equal(w1, w8)
+    md.close()
+    op.close()
+
+
+def test_sliced_layout_in_a_multi_run_and_its_limits():
+    """Single-shot runs in the sliced layout step inside a multi-run launch
+    beside a 64-shot run, bit-identical to their separate runs (FP64); the
+    layout refuses lane groups of more than one shot and windows that are not
+    a multiple of 64 rows."""
+    ca = _case(host.Q4, 14, 70, nshots=70)
+    cb = _case(host.Q9, 8, 70)
+    say("sliced layout in a multi-run: Q4 14x14 x 70 shots (sp 64) + Q9 8x8 (variant 8)")
+    keep, sep = [], []
+    runs = []
+    for c, kw in ((ca, dict(shots_per_lane_group=64, kernel_variant=3)), (cb, dict(kernel_variant=8))):
+        p = c["problem"]
+        op = gpu.Operator(p["node_num"], p["csr"])
+        md = gpu.Medium(op, p["c"], p["mass_lump"], p["dirichlet"], p["mpml"])
+        keep += [op, md]
+        mk = lambda: gpu.Run(op, md, c["src"], 70, c["rec"], c["dt"], c["f0"], c["t0"], precision=gpu.FP64,
+                             block=256, **kw)
+        runs.append(mk())
+        r = mk()
+        r.advance(70)
+        sep.append(r.seismograms())
+        r.close()
+    grp = gpu.Multi(runs)
+    grp.advance(70)
+    for r, (u, w) in zip(runs, sep):
+        gu, gw = r.seismograms()
+        assert bits_equal(gu, u) and bits_equal(gw, w)
+    grp.close()
+    for r in runs:
+        r.close()
+    op, md = keep[2], keep[3]
+    for kw in (dict(shots_per_lane_group=4, kernel_variant=8), dict(kernel_variant=8, sell_sigma=100)):
+        with pytest.raises(gpu.NativeError) as ei:
+            gpu.Run(op, md, cb["src"], 10, cb["rec"], cb["dt"], cb["f0"], cb["t0"], **kw)
+        assert ei.value.code == -1
+    for h in keep:
+        h.close()
+
+
 def test_example6_on_gpu_character_identical():
     from test_oracle import ex6_problem
     par, prob, src, rec = ex6_problem()
