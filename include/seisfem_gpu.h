@@ -162,8 +162,9 @@ typedef struct {
     const int *row_order;      /* [node_num] permutation: the order in which the step kernel's
                                   waves take rows (e.g. sfh_locality_order), NULL = node order.
                                   Changes only which rows run together, not any result */
-    int sell_sigma;            /* kernel_variant 8: rows per sorting window, a multiple of 64
-                                  (64 = no sorting), 0 = 128 */
+    int sell_sigma;            /* kernel_variants 8 and 9: rows per sorting window, a multiple of
+                                  64 (dividing the block for 9); 0 = 64, each slice of 64 rows
+                                  sorted on its own */
 } sfg_shot_desc;
 
 /* Creates a run with all state zeroed (elastic_wave.c:466-502). */

@@ -38,10 +38,10 @@ def candidates(nshots, nodes=160801, patches=False):
         # patches of the mesh (9)
         cands = [c(1, b, v, x, o) for b in (64, 128) for v in (1, 5) for x in (True, False)
                  for o in ("node", "hilbert")]
-        cands += [c(1, b, 8, x, "node", sg) for b in (128, 256) for x in (True, False) for sg in (0, 64)]
+        cands += [c(1, b, 8, x, "node", sg) for b in (128, 256) for x in (True, False) for sg in (0, 128)]
         if patches:
-            cands += [c(1, b, 9, x, o, sg) for b, o in ((256, "patch16x16"), (256, "patch32x8"), (64, "patch16x4"))
-                      for x in (True, False) for sg in (0, 64)]
+            cands += [c(1, b, 9, x, o) for b, o in ((256, "patch16x16"), (256, "patch32x8"), (64, "patch16x4"))
+                      for x in (True, False)]
         return cands
     if nshots <= 4:
         sps, blocks = [4, 16], [256]

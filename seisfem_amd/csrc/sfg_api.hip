@@ -732,7 +732,7 @@ static int run_create_impl(sfg_run **out, const sfg_operator *op, const sfg_medi
     if (d->kernel_variant == 8 || d->kernel_variant == 9) {
         const bool staged = d->kernel_variant == 9;
         const int R = r->block;                                  // rows per block (one per lane)
-        const int win = d->sell_sigma > 0 ? d->sell_sigma : staged ? R : 128;
+        const int win = d->sell_sigma > 0 ? d->sell_sigma : 64;
         if (sp != 1 || win % 64 || (staged && R % win)) {
             run_free(r);
             return fail(SFG_E_ARG, "run: kernel_variant %d (sliced layout) needs one shot per lane group and a "

@@ -458,7 +458,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
             constexpr int U = decltype(per_trip)::value;
             if constexpr (SELL) {
                 // the slots' loads run SELL_AHEAD slots ahead of their use (a ring of
-                // registers), so a wave keeps several slots' loads in flight
+                // registers), so a wave keeps several slots' loads in flight.  Every
+                // lane walks to the slice's longest row (its padded slots add exact
+                // zeros): bounding each lane by its own row measured slower, the
+                // divergent ring costing more than the padded lines it skips.
                 constexpr int PA = SELL_AHEAD;
                 Slot ring[PA];
 #pragma unroll
