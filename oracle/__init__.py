@@ -16,7 +16,8 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+# ORACLE_LIB selects another build (the sanitized one of tests/sanitize.sh)
+ORACLE_LIB = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libsfem_ref.so")
 
 _i, _d = C.c_int, C.c_double

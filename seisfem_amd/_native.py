@@ -9,7 +9,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libseisfem_amd.so")
+# SEISFEM_AMD_LIB selects another build of the same library (the sanitized host
+# build of tests/sanitize.sh)
+LIB_PATH = os.environ.get("SEISFEM_AMD_LIB") or os.path.join(_HERE, "libseisfem_amd.so")
 
 _i = C.c_int
 _d = C.c_double
