@@ -1312,13 +1312,25 @@ extern "C" int sfg_estimate_dt(const sfg_operator *op, const sfg_medium *md, int
 // ---------------------------------------------------------------------------
 // multi-runs: several runs stepped by one kernel launch per step
 
+// One launch of a multi-run step: the runs it steps, their argument table and
+// the first tile of each (runs in order, tiles of a run = its row blocks x shot
+// groups, row blocks fastest).
+struct MultiGroup {
+    std::vector<sfg_run *> runs;
+    int ntiles = 0;
+    int xcd_remap = 1;          // the group's tile map: 0 round-robin, 1 contiguous, Q + 1 stripes of Q
+    size_t lds_bytes = 0;       // dynamic LDS of the launch (LDS-staged group)
+    void *args = nullptr;       // device StepArgs<T>[runs]
+    int *tile_start = nullptr;  // device [runs + 1]
+};
+
 struct sfg_multi {
     std::vector<sfg_run *> runs;
-    int prec = 0, block = 256, ntiles = 0, max_steps = 0, guard_every = 50, executed = 0;
-    int xcd_remap = 1;          // the first run's tile map: 0 round-robin, 1 contiguous, Q + 1 stripes of Q
+    int prec = 0, block = 256, max_steps = 0, guard_every = 50, executed = 0;
     Ctrl *ctrl = nullptr;       // shared step counter
-    void *args = nullptr;       // device StepArgs<T>[nruns]
-    int *tile_start = nullptr;  // device [nruns + 1]
+    // per step: one launch of the per-row and batched forms, one of the single-shot
+    // runs in the LDS-staged sliced layout (each only when it has runs)
+    MultiGroup grp[2];
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // by parity of the first step
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1334,15 +1346,22 @@ static void multi_free(sfg_multi *m)
     if (m->ev1) (void)hipEventDestroy(m->ev1);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     dfree(m->ctrl);
-    dfree(m->args);
-    dfree(m->tile_start);
+    for (MultiGroup &g : m->grp) {
+        dfree(g.args);
+        dfree(g.tile_start);
+    }
     delete m;
 }
 
 static int multi_launch_step(sfg_multi *m, int slot)
 {
-    HIPCHK(launch_multi_step(m->prec, m->args, m->tile_start, (int)m->runs.size(), m->ntiles, m->block, m->xcd_remap, slot,
-                             m->stream));
+    const MultiGroup &g0 = m->grp[0], &g1 = m->grp[1];
+    if (!g0.runs.empty())
+        HIPCHK(launch_multi_step(m->prec, g0.args, g0.tile_start, (int)g0.runs.size(), g0.ntiles, m->block,
+                                 g0.xcd_remap, slot, m->stream));
+    if (!g1.runs.empty())
+        HIPCHK(launch_multi_step_lds(m->prec, g1.args, g1.tile_start, (int)g1.runs.size(), g1.ntiles, m->block,
+                                     g1.xcd_remap, slot, g1.lds_bytes, m->stream));
     return SFG_OK;
 }
 
@@ -1376,43 +1395,70 @@ extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns
     m->block = r0->block;
     m->max_steps = r0->max_steps;
     m->guard_every = r0->guard_every;
-    {
-        const int remap = r0->prec == 0 ? r0->a64.xcd_remap : r0->a32.xcd_remap;
-        const int stripe = r0->prec == 0 ? r0->a64.xcd_stripe : r0->a32.xcd_stripe;
-        m->xcd_remap = stripe > 0 ? stripe + 1 : remap;
-    }
-    std::vector<int> ts(nruns + 1, 0);
+    // single-shot runs in the LDS-staged sliced layout step in a launch of their own
+    auto staged = [](const sfg_run *r) { return (r->prec == 0 ? r->a64.variant : r->a32.variant) == 8; };
     for (int k = 0; k < nruns; ++k) {
         m->runs.push_back(runs[k]);
-        ts[k + 1] = ts[k] + runs[k]->nbx * runs[k]->nby;
+        m->grp[staged(runs[k]) ? 1 : 0].runs.push_back(runs[k]);
     }
-    m->ntiles = ts[nruns];
-    const size_t asz = m->prec == 0 ? sizeof(StepArgs<double>) : sizeof(StepArgs<float>);
     int rc = SFG_OK;
-    if ((rc = dalloc(&m->ctrl, 1)) || (rc = dalloc((char **)&m->args, asz * nruns)) ||
-        (rc = dalloc(&m->tile_start, nruns + 1))) {
+    if ((rc = dalloc(&m->ctrl, 1))) {
         multi_free(m);
         return rc;
     }
-    // argument table: each run's own arguments with the shared step counter
-    std::vector<char> tab(asz * nruns);
-    for (int k = 0; k < nruns; ++k) {
-        if (m->prec == 0) {
-            StepArgs<double> a = runs[k]->a64;
-            a.ctrl = m->ctrl;
-            std::memcpy(tab.data() + asz * k, &a, asz);
-        } else {
-            StepArgs<float> a = runs[k]->a32;
-            a.ctrl = m->ctrl;
-            std::memcpy(tab.data() + asz * k, &a, asz);
+    const size_t asz = m->prec == 0 ? sizeof(StepArgs<double>) : sizeof(StepArgs<float>);
+    std::vector<std::vector<char>> tabs(2);
+    std::vector<std::vector<int>> tss(2);
+    for (int gi = 0; gi < 2; ++gi) {
+        MultiGroup &g = m->grp[gi];
+        const int n = (int)g.runs.size();
+        if (n == 0) continue;
+        {
+            // the group's first run sets its tile map
+            const sfg_run *f = g.runs[0];
+            const int remap = f->prec == 0 ? f->a64.xcd_remap : f->a32.xcd_remap;
+            const int stripe = f->prec == 0 ? f->a64.xcd_stripe : f->a32.xcd_stripe;
+            g.xcd_remap = stripe > 0 ? stripe + 1 : remap;
+        }
+        std::vector<int> &ts = tss[gi];
+        ts.assign(n + 1, 0);
+        for (int k = 0; k < n; ++k) {
+            ts[k + 1] = ts[k] + g.runs[k]->nbx * g.runs[k]->nby;
+            g.lds_bytes = std::max(g.lds_bytes, g.runs[k]->lds_bytes);
+        }
+        g.ntiles = ts[n];
+        if ((rc = dalloc((char **)&g.args, asz * n)) || (rc = dalloc(&g.tile_start, n + 1))) {
+            multi_free(m);
+            return rc;
+        }
+        // argument table: each run's own arguments with the shared step counter
+        std::vector<char> &tab = tabs[gi];
+        tab.resize(asz * n);
+        for (int k = 0; k < n; ++k) {
+            if (m->prec == 0) {
+                StepArgs<double> a = g.runs[k]->a64;
+                a.ctrl = m->ctrl;
+                std::memcpy(tab.data() + asz * k, &a, asz);
+            } else {
+                StepArgs<float> a = g.runs[k]->a32;
+                a.ctrl = m->ctrl;
+                std::memcpy(tab.data() + asz * k, &a, asz);
+            }
         }
     }
     hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e == hipSuccess && !m->grp[1].runs.empty()) e = allow_multi_step_lds(m->prec, m->grp[1].lds_bytes);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
     if (e == hipSuccess) e = hipEventCreate(&m->ev1);
     if (e == hipSuccess) e = launch_fill_zero(m->ctrl, sizeof(Ctrl), m->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(m->args, tab.data(), tab.size(), hipMemcpyHostToDevice, m->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(m->tile_start, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice, m->stream);
+    for (int gi = 0; gi < 2; ++gi) {
+        if (m->grp[gi].runs.empty()) continue;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(m->grp[gi].args, tabs[gi].data(), tabs[gi].size(), hipMemcpyHostToDevice, m->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(m->grp[gi].tile_start, tss[gi].data(), sizeof(int) * tss[gi].size(),
+                               hipMemcpyHostToDevice, m->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
     if (e != hipSuccess) {
         multi_free(m);

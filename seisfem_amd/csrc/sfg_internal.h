@@ -152,6 +152,10 @@ hipError_t launch_step(int precision, int sp, bool coef_per_shot, const void *ar
 hipError_t allow_step_lds(int precision, int sp, bool coef_per_shot, size_t lds_bytes);
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
                              int block, int xcd_remap, int slot, hipStream_t st);
+// multi-run launch of single-shot runs in the LDS-staged sliced layout (kernel variant 8)
+hipError_t launch_multi_step_lds(int precision, const void *args_dev, const int *tile_start_dev, int nruns,
+                                 int ntiles, int block, int xcd_remap, int slot, size_t lds_bytes, hipStream_t st);
+hipError_t allow_multi_step_lds(int precision, size_t lds_bytes);
 // uw (FP32 forms, may be null): U and W are read from it instead of being summed
 // from the split fields.  shot_steps[s] (s < L.S): steps shot s executes; a shot
 // that has finished is read from the buffer of its own last step

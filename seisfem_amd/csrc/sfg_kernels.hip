@@ -789,6 +789,61 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
 #undef SFG_T
 }
 
+// The single-shot runs of a multi-run in the LDS-staged sliced layout (kernel
+// variant 8) get a launch of their own per step: their tiles run at this
+// form's register count and take its dynamic LDS (the largest halo of these
+// runs), not the batched forms' register count.  Same tile table scheme as
+// multi_step_kernel; every run shares args[0].ctrl.
+template <class T, int PREC>
+__global__ __launch_bounds__(256) void multi_step_lds_kernel(const StepArgs<T> *__restrict__ args,
+                                                             const int *__restrict__ tile_start, int nruns,
+                                                             int xcd_remap, int slot)
+{
+    Ctrl *ctrl = args[0].ctrl;
+    const int t = begin_step(ctrl, slot);
+    const unsigned int nb = gridDim.x;
+    unsigned int bid = blockIdx.x;
+    if (xcd_remap > 1) bid = xcd_striped(bid, nb, (unsigned int)(xcd_remap - 1));
+    else if (xcd_remap) bid = xcd_contiguous(bid, nb);
+    int r = 0;
+    while (r + 1 < nruns && (int)bid >= tile_start[r + 1]) ++r;
+    const StepArgs<T> &a = args[r];
+    const int local = (int)bid - tile_start[r];
+    const int bx = local % a.nbx, by = local / a.nbx;
+    if (a.cps) step_tile<T, PREC, 1, true, 8>(a, t, bx, by);
+    else step_tile<T, PREC, 1, false, 8>(a, t, bx, by);
+}
+
+hipError_t launch_multi_step_lds(int precision, const void *args_dev, const int *tile_start_dev, int nruns,
+                                 int ntiles, int block, int xcd_remap, int slot, size_t lds_bytes, hipStream_t st)
+{
+#define SFG_ML(T, P)                                                                                        \
+    hipLaunchKernelGGL((multi_step_lds_kernel<T, P>), dim3(ntiles), dim3(block), lds_bytes, st,            \
+                       (const StepArgs<T> *)args_dev, tile_start_dev, nruns, xcd_remap, slot)
+    switch (precision) {
+    case 0: SFG_ML(double, 0); break;
+    case 1: SFG_ML(float, 1); break;
+    case 2: SFG_ML(float, 2); break;
+    case 3: SFG_ML(float, 3); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef SFG_ML
+    return hipGetLastError();
+}
+
+hipError_t allow_multi_step_lds(int precision, size_t lds_bytes)
+{
+    const void *f = nullptr;
+    switch (precision) {
+    case 0: f = (const void *)multi_step_lds_kernel<double, 0>; break;
+    case 1: f = (const void *)multi_step_lds_kernel<float, 1>; break;
+    case 2: f = (const void *)multi_step_lds_kernel<float, 2>; break;
+    case 3: f = (const void *)multi_step_lds_kernel<float, 3>; break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
                              int block, int xcd_remap, int slot, hipStream_t st)
 {

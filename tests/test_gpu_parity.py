@@ -184,16 +184,22 @@ def test_sliced_layout_fp32_forms_equal_row_layout(prec):
 
 
 def test_sliced_layout_in_a_multi_run_and_its_limits():
-    """Single-shot runs in the sliced layout step inside a multi-run launch
-    beside a 64-shot run, bit-identical to their separate runs (FP64); the
-    layout refuses lane groups of more than one shot and windows that are not
-    a multiple of 64 rows."""
+    """Single-shot runs in the sliced layouts step inside a multi-run beside a
+    64-shot run (variant 8 in the shared launch, variant 9 in a launch of its
+    own each step), bit-identical to their separate runs (FP64); the layouts
+    refuse lane groups of more than one shot and windows that are not a
+    multiple of 64 rows (or do not divide the block, for variant 9)."""
     ca = _case(host.Q4, 14, 70, nshots=70)
     cb = _case(host.Q9, 8, 70)
-    say("sliced layout in a multi-run: Q4 14x14 x 70 shots (sp 64) + Q9 8x8 (variant 8)")
+    cc = _case(host.T6, 9, 70)
+    cd = _case(host.Q4, 17, 70)
+    say("sliced layouts in a multi-run: Q4 14x14 x 70 shots (sp 64) + Q9 8x8 (variant 8) + T6 9x9 and Q4 17x17 "
+        "(variant 9, a launch of their own)")
     keep, sep = [], []
     runs = []
-    for c, kw in ((ca, dict(shots_per_lane_group=64, kernel_variant=3)), (cb, dict(kernel_variant=8))):
+    patch = lambda c: host.patch_order(c["problem"]["x"], c["problem"]["y"], 16, 16)
+    for c, kw in ((ca, dict(shots_per_lane_group=64, kernel_variant=3)), (cb, dict(kernel_variant=8)),
+                  (cc, dict(kernel_variant=9, row_order=patch(cc))), (cd, dict(kernel_variant=9, row_order=patch(cd)))):
         p = c["problem"]
         op = gpu.Operator(p["node_num"], p["csr"])
         md = gpu.Medium(op, p["c"], p["mass_lump"], p["dirichlet"], p["mpml"])
@@ -214,7 +220,8 @@ def test_sliced_layout_in_a_multi_run_and_its_limits():
     for r in runs:
         r.close()
     op, md = keep[2], keep[3]
-    for kw in (dict(shots_per_lane_group=4, kernel_variant=8), dict(kernel_variant=8, sell_sigma=100)):
+    for kw in (dict(shots_per_lane_group=4, kernel_variant=8), dict(kernel_variant=8, sell_sigma=100),
+               dict(kernel_variant=9, block=128, sell_sigma=192)):
         with pytest.raises(gpu.NativeError) as ei:
             gpu.Run(op, md, cb["src"], 10, cb["rec"], cb["dt"], cb["f0"], cb["t0"], **kw)
         assert ei.value.code == -1
