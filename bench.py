@@ -343,12 +343,11 @@ def main():
             mop = gpu.Operator(p["node_num"], p["csr"])
             mmd = gpu.Medium(mop, p["c"], p["mass_lump"], p["dirichlet"], p["mpml"])
             keep += [mop, mmd]
-            # single-shot models in the sliced layout with LDS-staged halos over patches of
-            # their mesh, stepped by a launch of their own in the same graph
-            blk = kw.get("block", 256)
+            # single-shot models in the sliced layout (round-robin tiles), stepped by a
+            # launch of single-shot forms of their own, after part 1's own step launch
             runs.append(gpu.Run(mop, mmd, c["src"], L, c["rec"], c["dt"], c["f0"], c["t0"], precision=prec,
-                                nsteps=c["nsteps"], block=blk, shots_per_lane_group=1, kernel_variant=9,
-                                row_order=host.patch_order(p["x"], p["y"], 32, blk // 32)))
+                                nsteps=c["nsteps"], block=kw.get("block", 256), shots_per_lane_group=1,
+                                kernel_variant=8, xcd_remap=False))
         grp = gpu.Multi(runs)
         grp.advance(L)
         ms_m = grp.last_ms_per_step
@@ -376,8 +375,8 @@ def main():
                  "achieved_GBs": bytes_m / (ms_m * 1e-3) / 1e9, "frac": bytes_m / (ms_m * 1e-3) / 1e9 / HBM_PEAK_GBS,
                  "sizes": [list(c["size"]) for c in models], "max_rel_l2_vs_fp64": max(errs),
                  "part1_identical_to_single_run": p1_same,
-                 "launch": "per step one launch for part 1 (128 shots) and one for the 16 single-shot models "
-                           "(sliced layout, LDS-staged halos), both in one hipGraph"}
+                 "launch": "per step part 1's own step launch (128 shots) and one launch of the 16 "
+                           "single-shot models (sliced layout), both in one hipGraph"}
         log(f"multi-model launch: {ms_m:.3f} ms/launch, {multi['value'] / 1e9:.2f} G DOF-steps/s, "
             f"{multi['achieved_GBs']:.0f} GB/s, rel-L2 {max(errs):.2e}")
         grp.close()

@@ -216,14 +216,17 @@ void sfg_run_destroy(sfg_run *run);
 /* ---- several runs in one launch per step ---------------------------------- */
 
 /* Groups runs on different meshes / media / shot counts (e.g. many models of
- * uneven size and step count) so that one kernel launch steps all of them;
- * the launches of guard_every steps are captured in one hipGraph.  The runs
- * must share precision, max_steps (use per-shot nsteps for uneven lengths),
- * block size and guard interval, have lane groups of 1, 32 or 64 shots, and not
- * have stepped yet; the group's launch follows the first run's xcd_remap.
- * While grouped, a run is advanced only through the group;
- * its seismograms and snapshots are read as usual.  No counterpart in the
- * reference, which runs one model per process. */
+ * uneven size and step count) so that they step together: each step, every
+ * batched run large enough to fill the GPU alone takes its own step launch,
+ * the smaller batched runs (32 or 64 shots per lane group) share one launch,
+ * and the single-shot runs share one launch per layout (kernel variant 9 in a
+ * launch of its own, with the largest halo's LDS); all of them read one device
+ * step counter and the launches of guard_every steps are captured in one
+ * hipGraph.  The runs must share precision, max_steps (use per-shot nsteps for
+ * uneven lengths), block size and guard interval and not have stepped yet.
+ * While grouped, a run is advanced only through the group; its seismograms
+ * and snapshots are read as usual.  No counterpart in the reference, which
+ * runs one model per process. */
 typedef struct sfg_multi sfg_multi;
 int sfg_multi_create(sfg_multi **multi, sfg_run *const *runs, int nruns);
 /* Executes up to nsteps further steps of every run; returns the step count so far. */

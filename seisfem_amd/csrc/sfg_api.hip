@@ -1328,9 +1328,17 @@ struct sfg_multi {
     std::vector<sfg_run *> runs;
     int prec = 0, block = 256, max_steps = 0, guard_every = 50, executed = 0;
     Ctrl *ctrl = nullptr;       // shared step counter
-    // per step: one launch of the per-row and batched forms, one of the single-shot
-    // runs in the LDS-staged sliced layout (each only when it has runs)
-    MultiGroup grp[2];
+    // Per step: each batched run (several shots per lane group) that fills the
+    // chip on its own takes its own step launch (the single-run kernel, with its
+    // compaction of finished shot groups), reading the shared counter; then one
+    // launch of the smaller batched runs (grp[0]), one of the single-shot runs in
+    // the per-row or sliced layouts (grp[1], a kernel with only those forms) and
+    // one of the single-shot runs with LDS-staged halos (grp[2]), each only when
+    // it has runs.
+    std::vector<sfg_run *> solo;
+    std::vector<StepArgs<double>> solo64;   // their arguments with the shared counter
+    std::vector<StepArgs<float>> solo32;
+    MultiGroup grp[3];
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // by parity of the first step
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1355,13 +1363,21 @@ static void multi_free(sfg_multi *m)
 
 static int multi_launch_step(sfg_multi *m, int slot)
 {
-    const MultiGroup &g0 = m->grp[0], &g1 = m->grp[1];
-    if (!g0.runs.empty())
-        HIPCHK(launch_multi_step(m->prec, g0.args, g0.tile_start, (int)g0.runs.size(), g0.ntiles, m->block,
-                                 g0.xcd_remap, slot, m->stream));
-    if (!g1.runs.empty())
-        HIPCHK(launch_multi_step_lds(m->prec, g1.args, g1.tile_start, (int)g1.runs.size(), g1.ntiles, m->block,
-                                     g1.xcd_remap, slot, g1.lds_bytes, m->stream));
+    for (size_t k = 0; k < m->solo.size(); ++k) {
+        const sfg_run *r = m->solo[k];
+        const void *args = m->prec == 0 ? (const void *)&m->solo64[k] : (const void *)&m->solo32[k];
+        HIPCHK(launch_step(r->prec, r->SP, r->cps, args, r->nbx, r->nby, r->block, slot, r->lds_bytes, m->stream));
+    }
+    for (int gi = 0; gi < 2; ++gi) {
+        const MultiGroup &g = m->grp[gi];
+        if (!g.runs.empty())
+            HIPCHK(launch_multi_step(m->prec, g.args, g.tile_start, (int)g.runs.size(), g.ntiles, m->block,
+                                     g.xcd_remap, slot, gi == 1, m->stream));
+    }
+    const MultiGroup &g2 = m->grp[2];
+    if (!g2.runs.empty())
+        HIPCHK(launch_multi_step_lds(m->prec, g2.args, g2.tile_start, (int)g2.runs.size(), g2.ntiles, m->block,
+                                     g2.xcd_remap, slot, g2.lds_bytes, m->stream));
     return SFG_OK;
 }
 
@@ -1387,29 +1403,39 @@ extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns
         if (r->prec != r0->prec || r->max_steps != r0->max_steps || r->block != r0->block ||
             r->guard_every != r0->guard_every)
             return fail(SFG_E_ARG, "multi: runs must share precision, max_steps, block and guard interval");
-        if (r->SP != 1 && r->SP != 32 && r->SP != 64)
-            return fail(SFG_E_ARG, "multi: runs need lane groups of 1, 32 or 64 shots");
     }
     sfg_multi *m = new sfg_multi;
     m->prec = r0->prec;
     m->block = r0->block;
     m->max_steps = r0->max_steps;
     m->guard_every = r0->guard_every;
-    // single-shot runs in the LDS-staged sliced layout step in a launch of their own
+    // which launch steps each run (see sfg_multi)
     auto staged = [](const sfg_run *r) { return (r->prec == 0 ? r->a64.variant : r->a32.variant) == 8; };
     for (int k = 0; k < nruns; ++k) {
-        m->runs.push_back(runs[k]);
-        m->grp[staged(runs[k]) ? 1 : 0].runs.push_back(runs[k]);
+        sfg_run *r = runs[k];
+        m->runs.push_back(r);
+        if (r->SP == 1) m->grp[staged(r) ? 2 : 1].runs.push_back(r);
+        else if ((r->SP == 32 || r->SP == 64) && r->nbx * r->nby < 256 && r->lds_bytes == 0) m->grp[0].runs.push_back(r);
+        else m->solo.push_back(r);
     }
     int rc = SFG_OK;
     if ((rc = dalloc(&m->ctrl, 1))) {
         multi_free(m);
         return rc;
     }
+    for (sfg_run *r : m->solo) {
+        if (m->prec == 0) {
+            m->solo64.push_back(r->a64);
+            m->solo64.back().ctrl = m->ctrl;
+        } else {
+            m->solo32.push_back(r->a32);
+            m->solo32.back().ctrl = m->ctrl;
+        }
+    }
     const size_t asz = m->prec == 0 ? sizeof(StepArgs<double>) : sizeof(StepArgs<float>);
-    std::vector<std::vector<char>> tabs(2);
-    std::vector<std::vector<int>> tss(2);
-    for (int gi = 0; gi < 2; ++gi) {
+    std::vector<std::vector<char>> tabs(3);
+    std::vector<std::vector<int>> tss(3);
+    for (int gi = 0; gi < 3; ++gi) {
         MultiGroup &g = m->grp[gi];
         const int n = (int)g.runs.size();
         if (n == 0) continue;
@@ -1447,11 +1473,11 @@ extern "C" int sfg_multi_create(sfg_multi **out, sfg_run *const *runs, int nruns
         }
     }
     hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
-    if (e == hipSuccess && !m->grp[1].runs.empty()) e = allow_multi_step_lds(m->prec, m->grp[1].lds_bytes);
+    if (e == hipSuccess && !m->grp[2].runs.empty()) e = allow_multi_step_lds(m->prec, m->grp[2].lds_bytes);
     if (e == hipSuccess) e = hipEventCreate(&m->ev0);
     if (e == hipSuccess) e = hipEventCreate(&m->ev1);
     if (e == hipSuccess) e = launch_fill_zero(m->ctrl, sizeof(Ctrl), m->stream);
-    for (int gi = 0; gi < 2; ++gi) {
+    for (int gi = 0; gi < 3; ++gi) {
         if (m->grp[gi].runs.empty()) continue;
         if (e == hipSuccess)
             e = hipMemcpyAsync(m->grp[gi].args, tabs[gi].data(), tabs[gi].size(), hipMemcpyHostToDevice, m->stream);

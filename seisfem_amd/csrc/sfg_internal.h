@@ -150,8 +150,9 @@ struct StepArgs {
 hipError_t launch_step(int precision, int sp, bool coef_per_shot, const void *args, int nblk_x,
                        int nblk_y, int block, int slot, size_t lds_bytes, hipStream_t st);
 hipError_t allow_step_lds(int precision, int sp, bool coef_per_shot, size_t lds_bytes);
+// single_shot_only: every run has one shot per lane group (a kernel with only those forms)
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
-                             int block, int xcd_remap, int slot, hipStream_t st);
+                             int block, int xcd_remap, int slot, bool single_shot_only, hipStream_t st);
 // multi-run launch of single-shot runs in the LDS-staged sliced layout (kernel variant 8)
 hipError_t launch_multi_step_lds(int precision, const void *args_dev, const int *tile_start_dev, int nruns,
                                  int ntiles, int block, int xcd_remap, int slot, size_t lds_bytes, hipStream_t st);

@@ -740,7 +740,8 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a, int slot
 // grid, row blocks fastest.  Runs with one row per lane (SP = 1) and runs with
 // 32- or 64-shot lane groups can be mixed; every run shares the step counter
 // of args[0].ctrl, and a run past its last step has no active shot.
-template <class T, int PREC>
+// SP1: only runs of one shot per lane group (their forms alone set the registers).
+template <class T, int PREC, bool SP1>
 __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__restrict__ args,
                                                          const int *__restrict__ tile_start, int nruns,
                                                          int xcd_remap, int slot)
@@ -774,16 +775,18 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
         else if (w) SFG_T(SPV, C, 1);                  \
         else SFG_T(SPV, C, 0);                         \
     } while (0)
-    if (a.sp == 1) {
+    if (SP1 || a.sp == 1) {
         // single-shot runs in the sliced layout (7) or the per-row layouts
         if (a.cps) { if (v == 7) SFG_T(1, true, 7); else if (w) SFG_T(1, true, 1); else SFG_T(1, true, 0); }
         else { if (v == 7) SFG_T(1, false, 7); else if (w) SFG_T(1, false, 1); else SFG_T(1, false, 0); }
-    } else if (a.sp == 32) {
-        if (a.cps) SFG_G(32, true);
-        else SFG_G(32, false);
-    } else {
-        if (a.cps) SFG_G(64, true);
-        else SFG_G(64, false);
+    } else if constexpr (!SP1) {
+        if (a.sp == 32) {
+            if (a.cps) SFG_G(32, true);
+            else SFG_G(32, false);
+        } else {
+            if (a.cps) SFG_G(64, true);
+            else SFG_G(64, false);
+        }
     }
 #undef SFG_G
 #undef SFG_T
@@ -844,24 +847,31 @@ hipError_t allow_multi_step_lds(int precision, size_t lds_bytes)
     return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
-hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
-                             int block, int xcd_remap, int slot, hipStream_t st)
+template <bool SP1>
+static hipError_t launch_multi_step_t(int precision, const void *args_dev, const int *tile_start_dev, int nruns,
+                                      int ntiles, int block, int xcd_remap, int slot, hipStream_t st)
 {
-    if (precision == 0)
-        hipLaunchKernelGGL((multi_step_kernel<double, 0>), dim3(ntiles), dim3(block), 0, st,
-                           (const StepArgs<double> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
-    else if (precision == 1)
-        hipLaunchKernelGGL((multi_step_kernel<float, 1>), dim3(ntiles), dim3(block), 0, st,
-                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
-    else if (precision == 2)
-        hipLaunchKernelGGL((multi_step_kernel<float, 2>), dim3(ntiles), dim3(block), 0, st,
-                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
-    else if (precision == 3)
-        hipLaunchKernelGGL((multi_step_kernel<float, 3>), dim3(ntiles), dim3(block), 0, st,
-                           (const StepArgs<float> *)args_dev, tile_start_dev, nruns, xcd_remap, slot);
-    else
-        return hipErrorInvalidValue;
+#define SFG_MM(T, P)                                                                                          \
+    hipLaunchKernelGGL((multi_step_kernel<T, P, SP1>), dim3(ntiles), dim3(block), 0, st,                      \
+                       (const StepArgs<T> *)args_dev, tile_start_dev, nruns, xcd_remap, slot)
+    switch (precision) {
+    case 0: SFG_MM(double, 0); break;
+    case 1: SFG_MM(float, 1); break;
+    case 2: SFG_MM(float, 2); break;
+    case 3: SFG_MM(float, 3); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef SFG_MM
     return hipGetLastError();
+}
+
+hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
+                             int block, int xcd_remap, int slot, bool single_shot_only, hipStream_t st)
+{
+    return single_shot_only
+               ? launch_multi_step_t<true>(precision, args_dev, tile_start_dev, nruns, ntiles, block, xcd_remap, slot, st)
+               : launch_multi_step_t<false>(precision, args_dev, tile_start_dev, nruns, ntiles, block, xcd_remap, slot,
+                                            st);
 }
 
 template <class T, int PREC, int V>

@@ -96,8 +96,8 @@ def test_config4a_batched_fp32_full_length_within_1e4(cfg4a, prec):
 
 def test_config4b_multi_launch_full_length(cfg4a):
     """Configuration 4 part 2 with part 1 in the same multi-run (FP32 + bfloat16
-    Kahan terms; the models in the LDS-staged sliced layout, stepped by their
-    own launch in the same graph, as the benchmark runs them): each of the 16
+    Kahan terms; part 1 in its own step launch, the models in the sliced
+    layout in a launch of their own, as the benchmark runs them): each of the 16
     models within 1e-4 of its own FP64 run over its full step count; part 1
     bit-identical to its separate run."""
     case, op, md, base, L, u64, w64 = cfg4a
@@ -120,7 +120,7 @@ def test_config4b_multi_launch_full_length(cfg4a):
         keep += [mop, mmd]
         runs32.append(gpu.Run(mop, mmd, c["src"], L, c["rec"], c["dt"], c["f0"], c["t0"],
                               precision=gpu.FP32_KAHAN16, nsteps=c["nsteps"], block=256, shots_per_lane_group=1,
-                              kernel_variant=9, row_order=host.patch_order(p["x"], p["y"], 32, 8)))
+                              kernel_variant=8, xcd_remap=False))
         runs64.append(gpu.Run(mop, mmd, c["src"], L, c["rec"], c["dt"], c["f0"], c["t0"], precision=gpu.FP64,
                               nsteps=c["nsteps"]))
     g32 = gpu.Multi(runs32)
