@@ -24,6 +24,8 @@ rest (sfg_run_reset) at the start of every pass.
   multi_model  configuration 4 part 2: the 128 shots plus 16 Q4 models of uneven size in one
                launch per step, each model checked against its FP64 run
   config2      Q9 400x400 VTI single shot (configuration 2): FP64 and FP32+Kahan step kernels
+  surrogate_network  configuration 5's network: Adam step time, inference rate, output-layer GEMM
+               TFLOP/s against the bf16 MFMA peak
   cpu_baseline the C restatement of the reference loop (oracle) on the host's physical cores,
                step loop timed alone, with and without the reference's %f text output
 
@@ -139,6 +141,44 @@ def config2_leg(args):
     return out
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
+
+
+def net_params(widths):
+    return sum(a * b + b for a, b in zip(widths[:-1], widths[1:]))
+
+
+def mlp_leg():
+    """Configuration 5's network (13 -> 1024 x 3 -> 32000, batch 256) on synthetic
+    data: Adam training-step time, bf16 inference rate, and the output layer's
+    forward GEMM (256 x 32000 x 1024) timed alone against the bf16 MFMA peak."""
+    from seisfem_amd import surrogate
+    rng = np.random.default_rng(0)
+    nin, hid, nh, nout, B = surrogate.NIN, 1024, 3, 32000, 256
+    X = rng.uniform(0.0, 1.0, (16 * B, nin)).astype(np.float32)
+    Y = rng.standard_normal((16 * B, nout)).astype(np.float32)
+    net = surrogate.MLP(nin, hid, nh, nout, batch=B, seed=1)
+    net.set_data(X, Y)
+    net.train(1, lr=1e-3)
+    _, ms = net.train(4, lr=1e-3)
+    _, pms = net.predict(X)
+    net.close()
+    widths = [nin] + [hid] * nh + [nout]
+    macs = sum(a * b for a, b in zip(widths[:-1], widths[1:]))
+    # forward 2 flops per MAC; backward: dW for every layer and dX for all but the first
+    train_flops = 2.0 * B * (3 * macs - widths[0] * widths[1])
+    g_ms = surrogate.gemm_bf16_time(B, nout, hid, reps=20)
+    g_tf = 2.0 * B * nout * hid / (g_ms * 1e-3) / 1e12
+    out = {"network": f"{nin}-{hid}x{nh}-{nout}, batch {B}, synthetic data", "params": net_params(widths),
+           "train_ms_per_step": ms, "train_TFLOPs": train_flops / (ms * 1e-3) / 1e12,
+           "predict_samples_per_s": X.shape[0] / (pms * 1e-3),
+           "output_gemm": {"shape": f"{B}x{nout}x{hid}", "ms": g_ms, "TFLOPs": g_tf,
+                           "frac": g_tf / MFMA_BF16_PEAK_TFLOPS}}
+    log(f"network: {ms:.3f} ms per Adam step, {out['predict_samples_per_s']:.3g} samples/s inference, "
+        f"output GEMM {g_ms * 1e3:.1f} us = {g_tf:.0f} TFLOP/s ({out['output_gemm']['frac']:.2f} of bf16 peak)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +204,8 @@ def main():
     ap.add_argument("--launch2", default="",
                     help="pin the config-2 launch of both forms (same form as --launch; no tuning)")
     ap.add_argument("--quick", action="store_true", help="small mesh and step counts (smoke of the bench itself)")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the configuration-5 network leg")
+    ap.add_argument("--mlp-only", action="store_true", help="only the configuration-5 network leg")
     ap.add_argument("--launch", default="",
                     help="pin the batched launch 'shots_per_lane_group,block,variant,xcd[,order[,window]]' (no "
                          "tuning); xcd: 1 contiguous ranges, 0 round-robin, Q > 1 stripes of Q row blocks; order: "
@@ -175,6 +217,9 @@ def main():
               "shots would shard across GPUs with no communication, but no multi-GPU figure is produced.",
               file=sys.stderr)
         sys.exit(2)
+    if args.mlp_only:
+        print(json.dumps({"surrogate_network": mlp_leg()}), flush=True)
+        return
     if args.config2_only:
         from seisfem_amd import gpu
         # the copy kernel whose byte counters calibrate the profiler's (FETCH_SIZE, WRITE_SIZE)
@@ -387,6 +432,7 @@ def main():
 
     # configuration 2: Q9 400x400 VTI, one shot, FP64 (reference arithmetic) and FP32+Kahan
     config2 = None if args.no_config2 else config2_leg(args)
+    network = None if (args.no_mlp or args.quick) else mlp_leg()
 
     # CPU baseline: the oracle (C restatement of the reference loop), bounded sample, child process
     cpu = None
@@ -451,6 +497,7 @@ def main():
         "accuracy_covers": None if acc is None else f"every executed step of every shot ({int(nsteps.min())}..{L})",
         "multi_model": multi,
         "config2": config2,
+        "surrogate_network": network,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

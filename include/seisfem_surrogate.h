@@ -41,6 +41,10 @@ int sfg_mlp_gradients(sfg_mlp *m, const float *Xb, const float *Yb, double *loss
 int sfg_mlp_get_layer(sfg_mlp *m, int layer, float *W, float *b);
 /* C [M][N] = bf16(A) [M][K] . bf16(B)^T [K][N] with FP32 accumulation (kernel test entry). */
 int sfg_gemm_bf16_nt(int M, int N, int K, const float *A, const float *B, float *C);
+/* Times `reps` launches of the same GEMM on device-resident bf16 operands with
+   pseudo-random entries (K a multiple of 32): *ms = average device time of one
+   launch (HIP events around the launches on their stream). */
+int sfg_gemm_bf16_time(int M, int N, int K, int reps, double *ms);
 
 #ifdef __cplusplus
 }

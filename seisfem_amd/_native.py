@@ -112,6 +112,7 @@ def lib():
         "sfg_mlp_gradients": (_i, [_vp, _pf, _pf, _pd, _i, _pf, _pf]),
         "sfg_mlp_get_layer": (_i, [_vp, _i, _pf, _pf]),
         "sfg_gemm_bf16_nt": (_i, [_i, _i, _i, _pf, _pf, _pf]),
+        "sfg_gemm_bf16_time": (_i, [_i, _i, _i, _i, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

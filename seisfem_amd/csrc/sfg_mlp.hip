@@ -54,12 +54,45 @@ __device__ __forceinline__ float gelu_grad(float x)
     return 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * du;
 }
 
-enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2 };
+// Adam hyper-parameters of one update, bias corrections included
+struct AdamArgs {
+    float lr, b1, b2, eps, bc1, bc2;
+};
+
+// one Adam update of weight w with gradient g; m and v are updated in place
+__device__ __forceinline__ float adam_update(float w, float g, float &m, float &v, const AdamArgs &a)
+{
+    const float mm = a.b1 * m + (1.0f - a.b1) * g;
+    const float vv = a.b2 * v + (1.0f - a.b2) * g * g;
+    m = mm;
+    v = vv;
+    return w - a.lr * (mm / a.bc1) / (sqrtf(vv / a.bc2) + a.eps);
+}
+
+// What the GEMM does with a finished element v = (A . B^T)[row][col] (C row stride ldc):
+//   EPI_STORE      C = v
+//   EPI_BIAS       C = v + bias[col]                 (H = bf16 of it, if given)
+//   EPI_BIAS_GELU  C = v + bias[col], H = bf16(gelu(C))
+//   EPI_MSE        d = v + bias[col] - T[tidx[row]][col]; C = scale d and H = bf16(scale d) on
+//                  columns < nvalid (0 beyond); *loss += sum d^2 (the mean-squared-error
+//                  gradient of the output layer, targets read in place by sample index)
+enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_MSE = 3 };
+struct EpiArgs {
+    int kind = EPI_STORE;
+    const float *bias = nullptr;
+    uint16_t *H = nullptr;
+    int ldh = 0;
+    const float *T = nullptr;
+    const int *tidx = nullptr;
+    int ldt = 0;
+    int nvalid = 0;
+    float scale = 0.0f;
+    double *loss = nullptr;
+};
 
 // ---- C = A . B^T, bf16 in, fp32 out -----------------------------------------
-// Block tile BM x BN (128 or 64), K in steps of BK = 64, NW waves (4 in 2 x 2,
-// or 8 in 4 x 2 for 128 x 128), each wave's part as 32 x 32 tiles of
-// v_mfma_f32_32x32x16_bf16.  The LDS
+// Block tile BM x BN, K in steps of BK = 64, NW waves in (NW / WN) x WN, each
+// wave's part as 32 x 32 tiles of v_mfma_f32_32x32x16_bf16.  The LDS
 // tiles are double-buffered: while a wave multiplies step k out of one buffer,
 // its share of step k+1 is loaded from global memory into registers, then
 // stored into the other buffer; one barrier per step.  blockIdx.z splits K
@@ -69,16 +102,14 @@ enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2 };
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int BK = 64, LDK = BK + 8;
 
-// TA: A is stored k-major ([K][M], lda >= M) instead of [M][K]; TB: B is stored
-// [K][N] instead of [N][K].  Such an operand is read along its contiguous index
-// in 16-byte pieces and transposed while it is written into the LDS tile.
-template <int BM, int BN, int NW, bool TA, bool TB>
+// TB: B is stored [K][N] (ldb >= N) instead of [N][K]; it is read along N in
+// 16-byte pieces and transposed while it is written into the LDS tile.
+template <int BM, int BN, int NW, int WN, bool TB>
 __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, int kchunk, const uint16_t *__restrict__ A,
                                                       int lda, const uint16_t *__restrict__ B, int ldb,
-                                                      float *__restrict__ C, int ldc, const float *__restrict__ bias,
-                                                      int epi, uint16_t *__restrict__ Hout, int ldh)
+                                                      float *__restrict__ C, int ldc, const EpiArgs ep)
 {
-    constexpr int WN = 2, WM = NW / WN;                 // waves along M and N
+    constexpr int WM = NW / WN;                         // waves along M and N
     constexpr int TM = BM / WM / 32, TN = BN / WN / 32; // 32 x 32 tiles per wave
     constexpr int NT = NW * 64;
     constexpr int LA = BM * BK / 8 / NT, LB = BN * BK / 8 / NT;  // 16-byte loads per thread and step
@@ -96,56 +127,82 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
         for (int b = 0; b < TN; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
-    uint4 ra[LA], rb[LB];
-    // 16-byte piece q of a tile row-major [rows][BK]: row q / 8, columns 8 (q % 8) ..
-    // piece q of an [rows][BK] tile: row q / 8, k 8 (q % 8) ..; of a k-major
-    // [BK][cols] tile: k q / (cols / 8), columns 8 (q % (cols / 8)) ..
-    auto load = [&](const uint16_t *X, int ld, int lim, int r0, int q, int k0, auto tcols) __attribute__((always_inline)) {
-        constexpr int TC = decltype(tcols)::value;      // 0: [rows][K], else k-major with TC columns
-        if constexpr (TC == 0) {
-            const int row = q >> 3, kk = k0 + (q & 7) * 8, g = r0 + row;
-            return (g < lim && kk < ke) ? *reinterpret_cast<const uint4 *>(X + (size_t)g * ld + kk)
-                                        : make_uint4(0, 0, 0, 0);
-        } else {
-            const int kk = k0 + q / (TC / 8), c = (q % (TC / 8)) * 8, g = r0 + c;
-            return (g < lim && kk < ke) ? *reinterpret_cast<const uint4 *>(X + (size_t)kk * ld + g)
-                                        : make_uint4(0, 0, 0, 0);
+    // Loads run two K steps ahead of the products: step t + 2 is fetched into
+    // register set t % 2 while step t is multiplied out of LDS buffer t % 2, and
+    // set (t + 1) % 2 (fetched a step earlier) is stored into the other buffer.
+    uint4 ra[2][LA], rb[2][LB];
+    // A (and B unless TB): 16-byte piece q of an [rows][BK] tile is row q / 8, k
+    // 8 (q % 8) ..  With TB the B tile [BK][BN] is read along N: thread t takes
+    // columns 8 (t % (BN / 8)) .. and the LB consecutive k rows LB (t / (BN / 8))
+    // .., and writes each column's LB values as one piece of the tile row.  In
+    // that tile the 16-byte chunk j of row n sits at chunk j ^ (n / 8 % 8), so
+    // that the 8 columns of a thread and their neighbours' spread over the banks.
+    auto load_rows = [&](const uint16_t *X, int ld, int lim, int r0, int q, int k0) __attribute__((always_inline)) {
+        const int row = q >> 3, kk = k0 + (q & 7) * 8, g = r0 + row;
+        return (g < lim && kk < ke) ? *reinterpret_cast<const uint4 *>(X + (size_t)g * ld + kk) : make_uint4(0, 0, 0, 0);
+    };
+    const int tcg = tid % (BN / 8), tkg = tid / (BN / 8);
+    auto fetch = [&](auto S, int k0) __attribute__((always_inline)) {
+        constexpr int st = decltype(S)::value;
+#pragma unroll
+        for (int i = 0; i < LA; ++i) ra[st][i] = load_rows(A, lda, M, m0, tid + NT * i, k0);
+#pragma unroll
+        for (int i = 0; i < LB; ++i) {
+            if constexpr (TB) {
+                const int kk = k0 + tkg * LB + i, g = n0 + tcg * 8;
+                rb[st][i] = (g < N && kk < ke) ? *reinterpret_cast<const uint4 *>(B + (size_t)kk * ldb + g)
+                                               : make_uint4(0, 0, 0, 0);
+            } else {
+                rb[st][i] = load_rows(B, ldb, N, n0, tid + NT * i, k0);
+            }
         }
     };
-    auto put = [&](uint16_t *T, int q, uint4 v, auto tcols) __attribute__((always_inline)) {
-        constexpr int TC = decltype(tcols)::value;
-        if constexpr (TC == 0) {
-            *reinterpret_cast<uint4 *>(&T[(q >> 3) * LDK + (q & 7) * 8]) = v;
-        } else {
-            const int k = q / (TC / 8), c = (q % (TC / 8)) * 8;
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    auto half = [](const uint4 &v, int e) __attribute__((always_inline)) {
+        const uint32_t w = e < 2 ? v.x : e < 4 ? v.y : e < 6 ? v.z : v.w;
+        return (uint32_t)((w >> (16 * (e & 1))) & 0xffffu);
+    };
+    auto stash = [&](auto S, int buf) __attribute__((always_inline)) {
+        constexpr int st = decltype(S)::value;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) T[(c + e) * LDK + k] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+        for (int i = 0; i < LA; ++i) {
+            const int q = tid + NT * i;
+            *reinterpret_cast<uint4 *>(&As[buf][(q >> 3) * LDK + (q & 7) * 8]) = ra[st][i];
+        }
+        if constexpr (TB) {
+            static_assert(LB == 1 || LB == 2 || LB == 4, "k-major piece");
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int kq = tkg * LB;
+                uint16_t *dst = &Bs[buf][(tcg * 8 + e) * LDK + (((kq >> 3) ^ (tcg & 7)) << 3) + (kq & 7)];
+                if constexpr (LB == 1) {
+                    *dst = (uint16_t)half(rb[st][0], e);
+                } else if constexpr (LB == 2) {
+                    *reinterpret_cast<uint32_t *>(dst) = half(rb[st][0], e) | (half(rb[st][1], e) << 16);
+                } else {
+                    *reinterpret_cast<uint2 *>(dst) = make_uint2(half(rb[st][0], e) | (half(rb[st][1], e) << 16),
+                                                                 half(rb[st][2], e) | (half(rb[st][3], e) << 16));
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < LB; ++i) {
+                const int q = tid + NT * i;
+                *reinterpret_cast<uint4 *>(&Bs[buf][(q >> 3) * LDK + (q & 7) * 8]) = rb[st][i];
+            }
         }
     };
-    using CA = std::integral_constant<int, TA ? BM : 0>;
-    using CB = std::integral_constant<int, TB ? BN : 0>;
-    auto fetch = [&](int k0) __attribute__((always_inline)) {
-#pragma unroll
-        for (int i = 0; i < LA; ++i) ra[i] = load(A, lda, M, m0, tid + NT * i, k0, CA{});
-#pragma unroll
-        for (int i = 0; i < LB; ++i) rb[i] = load(B, ldb, N, n0, tid + NT * i, k0, CB{});
-    };
-    auto stash = [&](int buf) __attribute__((always_inline)) {
-#pragma unroll
-        for (int i = 0; i < LA; ++i) put(As[buf], tid + NT * i, ra[i], CA{});
-#pragma unroll
-        for (int i = 0; i < LB; ++i) put(Bs[buf], tid + NT * i, rb[i], CB{});
-    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
     const int nk = (ke - kb + BK - 1) / BK;
     if (nk > 0) {
-        fetch(kb);
-        stash(0);
+        fetch(I0{}, kb);
+        if (nk > 1) fetch(I1{}, kb + BK);
+        stash(I0{}, 0);
     }
     __syncthreads();
-    for (int t = 0; t < nk; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < nk) fetch(kb + (t + 1) * BK);
+    auto step = [&](int t, auto S) __attribute__((always_inline)) {
+        constexpr int buf = decltype(S)::value;
+        if (t + 2 < nk) fetch(S, kb + (t + 2) * BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             // operands of v_mfma_f32_32x32x16_bf16: lane l holds row (col) l % 32 and
@@ -156,38 +213,92 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
             for (int a = 0; a < TM; ++a)
                 af[a] = *reinterpret_cast<const bf16x8 *>(&As[buf][(wm * (BM / WM) + a * 32 + (lane & 31)) * LDK + kof]);
 #pragma unroll
-            for (int b = 0; b < TN; ++b)
-                bfr[b] = *reinterpret_cast<const bf16x8 *>(&Bs[buf][(wn * (BN / WN) + b * 32 + (lane & 31)) * LDK + kof]);
+            for (int b = 0; b < TN; ++b) {
+                const int n = wn * (BN / WN) + b * 32 + (lane & 31);
+                const int kb_ = TB ? (((kof >> 3) ^ ((n >> 3) & 7)) << 3) : kof;
+                bfr[b] = *reinterpret_cast<const bf16x8 *>(&Bs[buf][n * LDK + kb_]);
+            }
 #pragma unroll
             for (int a = 0; a < TM; ++a)
 #pragma unroll
                 for (int b = 0; b < TN; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
         }
-        if (t + 1 < nk) stash(buf ^ 1);
+        if (t + 1 < nk) stash(std::integral_constant<int, 1 - buf>{}, 1 - buf);
         __syncthreads();
+    };
+    for (int t = 0; t < nk; t += 2) {
+        step(t, I0{});
+        if (t + 1 < nk) step(t + 1, I1{});
     }
-    // epilogue: lane l, register r -> row (r/4)*8 + (l/32)*4 + r%4, column l%32
-    const bool split = gridDim.z > 1;
+    // epilogue: lane l, register r -> row (r/4)*8 + (l/32)*4 + r%4, column l%32.
+    // Per 32 x 32 tile, every load an epilogue needs is issued before any of its
+    // stores (the stores could alias the loads, so loads interleaved with them
+    // would each wait out a full memory latency).  One unrolled walk over the
+    // tiles per epilogue kind keeps acc in registers.
+    auto tiles = [&](auto f) __attribute__((always_inline)) {
 #pragma unroll
-    for (int a = 0; a < TM; ++a)
+        for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < TN; ++b)
+            for (int b = 0; b < TN; ++b)
+                f(acc[a][b], n0 + wn * (BN / WN) + b * 32 + (lane & 31), m0 + wm * (BM / WM) + a * 32 + (lane >> 5) * 4);
+    };
+    // row of register r in a tile whose lane rows start at rb
+#define SFG_ROW(r) (rb + ((r) >> 2) * 8 + ((r) & 3))
+    if (gridDim.z > 1) {
+        tiles([&](const f32x16 &t, int col, int rb) __attribute__((always_inline)) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                if (col < N && SFG_ROW(r) < M) atomicAdd(&C[(size_t)SFG_ROW(r) * ldc + col], t[r]);
+        });
+    } else if (ep.kind == EPI_STORE) {
+        tiles([&](const f32x16 &t, int col, int rb) __attribute__((always_inline)) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                if (col < N && SFG_ROW(r) < M) C[(size_t)SFG_ROW(r) * ldc + col] = t[r];
+        });
+    } else if (ep.kind == EPI_MSE) {
+        double lsum = 0.0;
+        tiles([&](const f32x16 &t, int col, int rb) __attribute__((always_inline)) {
+            const bool cv = col < ep.nvalid;
+            const float bc = col < N ? ep.bias[col] : 0.0f;
+            int ti[16];
+            float tv[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) ti[r] = (cv && SFG_ROW(r) < M) ? ep.tidx[SFG_ROW(r)] : 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) tv[r] = (cv && SFG_ROW(r) < M) ? ep.T[(size_t)ti[r] * ep.ldt + col] : 0.0f;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int row = m0 + wm * (BM / WM) + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const int col = n0 + wn * (BN / WN) + b * 32 + (lane & 31);
-                if (row < M && col < N) {
-                    float v = acc[a][b][r];
-                    if (split) {
-                        atomicAdd(&C[(size_t)row * ldc + col], v);
-                        continue;
+                if (col < N && SFG_ROW(r) < M) {
+                    float g = 0.0f;
+                    if (cv) {
+                        const float d = (t[r] + bc) - tv[r];
+                        lsum += (double)d * d;
+                        g = ep.scale * d;
                     }
-                    if (epi != EPI_STORE) v += bias[col];
-                    C[(size_t)row * ldc + col] = v;
-                    if (Hout) Hout[(size_t)row * ldh + col] = f2bf(epi == EPI_BIAS_GELU ? gelu(v) : v);
+                    C[(size_t)SFG_ROW(r) * ldc + col] = g;
+                    ep.H[(size_t)SFG_ROW(r) * ep.ldh + col] = f2bf(g);
                 }
             }
+        });
+        for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
+        if (lane == 0) atomicAdd(ep.loss, lsum);
+    } else {
+        const bool act = ep.kind == EPI_BIAS_GELU;
+        tiles([&](const f32x16 &t, int col, int rb) __attribute__((always_inline)) {
+            const float bc = col < N ? ep.bias[col] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                if (col < N && SFG_ROW(r) < M) {
+                    const float v = t[r] + bc;
+                    C[(size_t)SFG_ROW(r) * ldc + col] = v;
+                    if (ep.H) ep.H[(size_t)SFG_ROW(r) * ep.ldh + col] = f2bf(act ? gelu(v) : v);
+                }
+            }
+        });
+    }
+#undef SFG_ROW
 }
 
 // C[r][c] = 0 for r < rows, c < cols (row stride ld)
@@ -234,35 +345,6 @@ __global__ void gather_rows_kernel(int batch, int width, int ld, const float *sr
     }
 }
 
-// dy = 2 (y - t) / (batch * nout) on valid columns (fp32 and bf16 copies); loss
-// partial sums (y-t)^2
-__global__ void mse_grad_kernel(int batch, int nout, int ld, const float *y, const float *t, float *dy,
-                                uint16_t *dy16, double *loss)
-{
-    __shared__ double red[256];
-    double acc = 0.0;
-    const size_t total = (size_t)batch * ld;
-    const float scale = 2.0f / ((float)batch * (float)nout);
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(q % ld);
-        float g = 0.0f;
-        if (c < nout) {
-            const float d = y[q] - t[q];
-            acc += (double)d * d;
-            g = scale * d;
-        }
-        dy[q] = g;
-        dy16[q] = f2bf(g);
-    }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
-}
-
 // dz = dh * gelu'(z) (fp32 and bf16 copies)
 __global__ void gelu_back_kernel(size_t n, const float *dh, const float *z, float *dz, uint16_t *dz16)
 {
@@ -275,8 +357,10 @@ __global__ void gelu_back_kernel(size_t n, const float *dh, const float *z, floa
 
 // db[c] = sum_b dz[b][c]: a block of 64 x 4 threads takes 64 columns, each of
 // its 4 rows of threads a quarter of the batch (4 independent partial sums in
-// flight), then the 4 partial sums meet in LDS
-__global__ __launch_bounds__(256) void bias_grad_kernel(int batch, int cols, int ld, const float *dz, float *db)
+// flight), then the 4 partial sums meet in LDS.  With bw given, the bias bw is
+// then updated by Adam (moments bm, bv) from that gradient.
+__global__ __launch_bounds__(256) void bias_grad_kernel(int batch, int cols, int ld, const float *dz, float *db,
+                                                        float *bw, float *bm, float *bv, AdamArgs ad)
 {
     __shared__ float part[4][64];
     const int cx = threadIdx.x & 63, ry = threadIdx.x >> 6;
@@ -293,25 +377,34 @@ __global__ __launch_bounds__(256) void bias_grad_kernel(int batch, int cols, int
     }
     part[ry][cx] = s;
     __syncthreads();
-    if (ry == 0 && c < cols) db[c] = (part[0][cx] + part[1][cx]) + (part[2][cx] + part[3][cx]);
+    if (ry == 0 && c < cols) {
+        const float g = (part[0][cx] + part[1][cx]) + (part[2][cx] + part[3][cx]);
+        db[c] = g;
+        if (bw) bw[c] = adam_update(bw[c], g, bm[c], bv[c], ad);
+    }
 }
 
-// Adam on FP32 master weights [rows][ld] (valid cols < cols); refreshes the bf16 W
-__global__ void adam_kernel(int rows, int cols, int ld, float *w, const float *g, float *m, float *v, float lr,
-                            float b1, float b2, float eps, float bc1, float bc2, uint16_t *w16)
+// Adam on FP32 master weights [rows][ld] (valid columns < cols, ld a multiple
+// of 4), four weights per thread and trip; refreshes the bf16 copy w16
+__global__ void adam_kernel(int rows, int cols, int ld, float *w, const float *g, float *m, float *v, AdamArgs a,
+                            uint16_t *w16)
 {
-    const size_t total = (size_t)rows * ld;
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-        const int r = (int)(q / ld), c = (int)(q % ld);
-        if (c >= cols) continue;
-        const float gr = g[q];
-        const float mm = b1 * m[q] + (1.0f - b1) * gr;
-        const float vv = b2 * v[q] + (1.0f - b2) * gr * gr;
-        m[q] = mm;
-        v[q] = vv;
-        const float wn = w[q] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
-        w[q] = wn;
-        if (w16) w16[q] = f2bf(wn);
+    const size_t total4 = (size_t)rows * ld / 4;
+    for (size_t q4 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q4 < total4; q4 += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)((q4 * 4) % ld);
+        float4 W = reinterpret_cast<float4 *>(w)[q4], Mm = reinterpret_cast<float4 *>(m)[q4];
+        float4 V = reinterpret_cast<float4 *>(v)[q4];
+        const float4 G = reinterpret_cast<const float4 *>(g)[q4];
+        float *pw = &W.x, *pm = &Mm.x, *pv = &V.x;
+        const float *pg = &G.x;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (c + e < cols) pw[e] = adam_update(pw[e], pg[e], pm[e], pv[e], a);
+        reinterpret_cast<float4 *>(w)[q4] = W;
+        reinterpret_cast<float4 *>(m)[q4] = Mm;
+        reinterpret_cast<float4 *>(v)[q4] = V;
+        reinterpret_cast<uint2 *>(w16)[q4] = make_uint2((uint32_t)f2bf(W.x) | ((uint32_t)f2bf(W.y) << 16),
+                                                        (uint32_t)f2bf(W.z) | ((uint32_t)f2bf(W.w) << 16));
     }
 }
 
@@ -334,21 +427,25 @@ int mfail(int code, const char *fmt, ...)
         if (e_ != hipSuccess) return mfail(SFG_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));     \
     } while (0)
 
-// C = A . B^T, A [M][K] (or, ta, [K][M]) and B [N][K] (or, tb, [K][N]).  128 x
-// 128 tiles when they give the GPU enough blocks, else 64 x 64; a plain store
-// (EPI_STORE) with still too few blocks splits K (the partial products meet in
-// C by atomics).
+// C = A . B^T, A [M][K] and B [N][K] (or, tb, [K][N]), epilogue ep.  Tiles of
+// 128 x 128 when they give the GPU enough blocks, else 64 x 64 (four waves of
+// 64 x 64 or 32 x 32); a plain store with still too few blocks splits K (the
+// partial products meet in C by atomics).
 hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float *C,
-                int ldc, const float *bias, int epi, uint16_t *H, int ldh, bool ta = false, bool tb = false)
+                int ldc, const EpiArgs &ep, bool tb = false)
 {
     if (K % 8 || lda % 8 || ldb % 8 || M <= 0 || N <= 0 || K <= 0) return hipErrorInvalidValue;
-    if ((ta && M % 8) || (tb && N % 8)) return hipErrorInvalidValue;
+    if (tb && N % 8) return hipErrorInvalidValue;
+    if (ep.kind == EPI_MSE && !ep.H) return hipErrorInvalidValue;
+    // 128 x 128 with four waves of 64 x 64 measured faster on 256 x 32000 x 1024
+    // than eight waves of 32 x 64 or 256 x 128 tiles; for the K = 32000 product
+    // (dX of the output layer) 64 x 64 tiles split 8 ways beat 256 x 128 split 63
     const bool big = (long)((M + 127) / 128) * ((N + 127) / 128) >= 256;
     const int bm = big ? 128 : 64, bn = bm;
     const long tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
     int splits = 1;
     const int ksteps = (K + BK - 1) / BK;
-    if (epi == EPI_STORE && !H && tiles < 256) splits = (int)std::min<long>(ksteps, (512 + tiles - 1) / tiles);
+    if (ep.kind == EPI_STORE && tiles < 256) splits = (int)std::min<long>(ksteps, (512 + tiles - 1) / tiles);
     const int kchunk = ((ksteps + splits - 1) / splits) * BK;
     splits = (K + kchunk - 1) / kchunk;
     if (splits > 1) {
@@ -356,15 +453,13 @@ hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int lda,
         hipLaunchKernelGGL(fill_zero_f32_kernel, dim3(1024), dim3(256), 0, st, M, N, ldc, C);
     }
     const dim3 g((N + bn - 1) / bn, (M + bm - 1) / bm, splits);
-#define SFG_GEMM(BMV, NWV, TAV, TBV)                                                                          \
-    hipLaunchKernelGGL((gemm_nt_kernel<BMV, BMV, NWV, TAV, TBV>), g, dim3(NWV * 64), 0, st, M, N, K, kchunk, A, \
-                       lda, B, ldb, C, ldc, bias, epi, H, ldh)
+#define SFG_GEMM(BMV, TBV)                                                                                        \
+    hipLaunchKernelGGL((gemm_nt_kernel<BMV, BMV, 4, 2, TBV>), g, dim3(256), 0, st, M, N, K, kchunk, A, lda, B, ldb, C, \
+                       ldc, ep)
     if (big) {
-        if (ta) { if (tb) SFG_GEMM(128, 8, true, true); else SFG_GEMM(128, 8, true, false); }
-        else { if (tb) SFG_GEMM(128, 8, false, true); else SFG_GEMM(128, 8, false, false); }
+        if (tb) SFG_GEMM(128, true); else SFG_GEMM(128, false);
     } else {
-        if (ta) { if (tb) SFG_GEMM(64, 4, true, true); else SFG_GEMM(64, 4, true, false); }
-        else { if (tb) SFG_GEMM(64, 4, false, true); else SFG_GEMM(64, 4, false, false); }
+        if (tb) SFG_GEMM(64, true); else SFG_GEMM(64, false);
     }
 #undef SFG_GEMM
     return hipGetLastError();
@@ -393,9 +488,10 @@ struct sfg_mlp {
     std::vector<uint16_t *> h16, hT16;
     std::vector<float *> z, dz, dh;
     std::vector<uint16_t *> dz16, dzT16;
-    float *y = nullptr, *t = nullptr;
+    float *y = nullptr;
     double *loss = nullptr;
-    int *idx = nullptr;
+    int *idx = nullptr;   // sample indices of one batch
+    int *order = nullptr; // an epoch's sample order (ndata)
     float *Xd = nullptr, *Yd = nullptr;
     int ndata = 0;
     long step = 0;
@@ -411,7 +507,7 @@ static void mlp_free(sfg_mlp *m)
     };
     F(m->W); F(m->G); F(m->Mw); F(m->Vw); F(m->b); F(m->gb); F(m->Mb); F(m->Vb);
     F(m->W16); F(m->h16); F(m->hT16); F(m->z); F(m->dz); F(m->dh); F(m->dz16); F(m->dzT16);
-    for (void *p : {(void *)m->y, (void *)m->t, (void *)m->loss, (void *)m->idx, (void *)m->Xd, (void *)m->Yd})
+    for (void *p : {(void *)m->y, (void *)m->loss, (void *)m->idx, (void *)m->order, (void *)m->Xd, (void *)m->Yd})
         if (p) (void)hipFree(p);
     if (m->st) (void)hipStreamDestroy(m->st);
     delete m;
@@ -487,7 +583,6 @@ extern "C" int sfg_mlp_create(sfg_mlp **out, int nin, int hidden, int nhidden_la
         m->dh.push_back(dhh); m->dz16.push_back(d16); m->dzT16.push_back(dT);
     }
     if (e == hipSuccess) e = zalloc(&m->y, (size_t)batch * m->pout.back());
-    if (e == hipSuccess) e = zalloc(&m->t, (size_t)batch * m->pout.back());
     if (e == hipSuccess) e = zalloc(&m->loss, 1);
     if (e == hipSuccess) e = zalloc(&m->idx, batch);
     // bf16 weights from the master copy
@@ -517,90 +612,114 @@ extern "C" long long sfg_mlp_param_count(const sfg_mlp *m)
     return c;
 }
 
-// forward of the current batch: h16[0] holds the input; y = output (fp32)
-static int forward(sfg_mlp *m)
+// forward of the current batch (h16[0] holds its input).  Without targets the
+// output goes to y; with targets Y (rows by the batch's sample indices idx, on
+// the device) the output layer's epilogue turns it into the loss gradient dz /
+// dz16 of the last layer and adds the squared errors to *loss (zeroed here).
+static int forward(sfg_mlp *m, const float *Y = nullptr, const int *idx = nullptr)
 {
+    if (Y) MCHK(hipMemsetAsync(m->loss, 0, sizeof(double), m->st));
     for (int l = 0; l < m->nl; ++l) {
         const bool last = l == m->nl - 1;
-        float *C = last ? m->y : m->z[l];
-        uint16_t *H = last ? nullptr : m->h16[l + 1];
+        EpiArgs ep;
+        ep.bias = m->b[l];
+        float *C = m->z[l];
+        if (!last) {
+            ep.kind = EPI_BIAS_GELU;
+            ep.H = m->h16[l + 1];
+            ep.ldh = m->pin[l + 1];
+        } else if (!Y) {
+            ep.kind = EPI_BIAS;
+            C = m->y;
+        } else {
+            ep.kind = EPI_MSE;
+            C = m->dz[l];
+            ep.H = m->dz16[l];
+            ep.ldh = m->pout[l];
+            ep.T = Y;
+            ep.tidx = idx;
+            ep.ldt = m->nout;
+            ep.nvalid = m->nout;
+            ep.scale = 2.0f / ((float)m->batch * (float)m->nout);
+            ep.loss = m->loss;
+        }
         MCHK(gemm(m->st, m->batch, m->pout[l], m->pin[l], m->h16[l], m->pin[l], m->W16[l], m->pin[l], C, m->pout[l],
-                  m->b[l], last ? EPI_BIAS : EPI_BIAS_GELU, H, last ? 0 : m->pin[l + 1]));
+                  ep));
     }
     return SFG_OK;
 }
 
-// backward from dz of the last layer (already in dz[nl-1] / dz16[nl-1]) -> G, gb
-static int backward(sfg_mlp *m)
+// backward from dz of the last layer (in dz[nl-1] / dz16[nl-1]): weight gradients
+// to G, bias gradients to gb.  With ad each layer's bias is also updated by Adam
+// as its gradient is formed, and its weights right after their gradient (a
+// layer's dX product reads its bf16 weights before they change).
+static int backward(sfg_mlp *m, const AdamArgs *ad = nullptr)
 {
     const int B = m->batch;
     for (int l = m->nl - 1; l >= 0; --l) {
+        if (l > 0) {
+            // dh = dz . W (W [pout][pin] read k-major as stored) ; dz_{l-1} = dh * gelu'(z_{l-1})
+            MCHK(gemm(m->st, B, m->pin[l], m->pout[l], m->dz16[l], m->pout[l], m->W16[l], m->pin[l], m->dh[l],
+                      m->pin[l], EpiArgs{}, true));
+            const size_t n = (size_t)B * m->pin[l];
+            hipLaunchKernelGGL(gelu_back_kernel, dim3(1024), dim3(256), 0, m->st, n, m->dh[l], m->z[l - 1],
+                               m->dz[l - 1], m->dz16[l - 1]);
+            MCHK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(bias_grad_kernel, dim3((m->pout[l] + 63) / 64), dim3(256), 0, m->st, B, m->dout[l],
+                           m->pout[l], m->dz[l], m->gb[l], ad ? m->b[l] : nullptr, m->Mb[l], m->Vb[l],
+                           ad ? *ad : AdamArgs{});
+        MCHK(hipGetLastError());
         // dW = dz^T . h from transposed copies (A = dz^T [pout][B], B = h^T [pin][B]); K = batch.
         // Reading both operands k-major instead measured 2.4x slower (transposing into LDS
         // element by element costs as much as the products at this small K)
         MCHK(transpose(m->st, B, m->pout[l], m->dz16[l], m->pout[l], m->dzT16[l], B));
         MCHK(transpose(m->st, B, m->pin[l], m->h16[l], m->pin[l], m->hT16[l], B));
-        MCHK(gemm(m->st, m->pout[l], m->pin[l], B, m->dzT16[l], B, m->hT16[l], B, m->G[l], m->pin[l], nullptr,
-                  EPI_STORE, nullptr, 0));
-        hipLaunchKernelGGL(bias_grad_kernel, dim3((m->pout[l] + 63) / 64), dim3(256), 0, m->st, B, m->dout[l],
-                           m->pout[l], m->dz[l], m->gb[l]);
-        MCHK(hipGetLastError());
-        if (l > 0) {
-            // dh = dz . W (W [pout][pin] read k-major as stored) ; dz_{l-1} = dh * gelu'(z_{l-1})
-            MCHK(gemm(m->st, B, m->pin[l], m->pout[l], m->dz16[l], m->pout[l], m->W16[l], m->pin[l], m->dh[l],
-                      m->pin[l], nullptr, EPI_STORE, nullptr, 0, false, true));
-            const size_t n = (size_t)B * m->pin[l];
-            hipLaunchKernelGGL(gelu_back_kernel, dim3(1024), dim3(256), 0, m->st, n, m->dh[l], m->z[l - 1],
-                               m->dz[l - 1], m->dz16[l - 1]);
+        MCHK(gemm(m->st, m->pout[l], m->pin[l], B, m->dzT16[l], B, m->hT16[l], B, m->G[l], m->pin[l], EpiArgs{}));
+        if (ad) {
+            const size_t n4 = (size_t)m->dout[l] * m->pin[l] / 4;
+            const int blocks = (int)std::min<size_t>(4096, (n4 + 255) / 256);
+            hipLaunchKernelGGL(adam_kernel, dim3(blocks), dim3(256), 0, m->st, m->dout[l], m->din[l], m->pin[l],
+                               m->W[l], m->G[l], m->Mw[l], m->Vw[l], *ad, m->W16[l]);
             MCHK(hipGetLastError());
         }
     }
     return SFG_OK;
 }
 
-static int load_batch(sfg_mlp *m, const int *idx_host, const float *X, const float *Y, int nsrc)
+// sample indices (host, range-checked) -> m->idx
+static int upload_idx(sfg_mlp *m, const int *idx_host, int nsrc)
 {
     if (nsrc <= 0) return mfail(SFG_E_ARG, "mlp: no data");
     for (int b = 0; b < m->batch; ++b)
         if (idx_host[b] < 0 || idx_host[b] >= nsrc) return mfail(SFG_E_ARG, "mlp: sample index out of range");
     MCHK(hipMemcpyAsync(m->idx, idx_host, sizeof(int) * m->batch, hipMemcpyHostToDevice, m->st));
+    return SFG_OK;
+}
+
+// rows idx[batch] (device) of X -> bf16 input h16[0]
+static int load_batch(sfg_mlp *m, const int *idx, const float *X)
+{
     hipLaunchKernelGGL(gather_rows_kernel, dim3(1, m->batch), dim3(64), 0, m->st, m->batch, m->nin, m->pin[0], X,
-                       m->idx, m->h16[0], (float *)nullptr);
-    if (Y)
-        hipLaunchKernelGGL(gather_rows_kernel, dim3(64, m->batch), dim3(256), 0, m->st, m->batch, m->nout,
-                           m->pout.back(), Y, m->idx, (uint16_t *)nullptr, m->t);
+                       idx, m->h16[0], (float *)nullptr);
     MCHK(hipGetLastError());
     return SFG_OK;
 }
 
-static int loss_grad(sfg_mlp *m, double *loss_out)
+// Adam's step count and bias corrections for the next update
+static AdamArgs next_adam(sfg_mlp *m, float lr)
 {
-    const int l = m->nl - 1;
-    MCHK(hipMemsetAsync(m->loss, 0, sizeof(double), m->st));
-    hipLaunchKernelGGL(mse_grad_kernel, dim3(1024), dim3(256), 0, m->st, m->batch, m->nout, m->pout[l], m->y, m->t,
-                       m->dz[l], m->dz16[l], m->loss);
-    MCHK(hipGetLastError());
-    if (loss_out) {
-        double h = 0.0;
-        MCHK(hipMemcpyAsync(&h, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->st));
-        MCHK(hipStreamSynchronize(m->st));
-        *loss_out = h / ((double)m->batch * m->nout);
-    }
-    return SFG_OK;
-}
-
-static int adam(sfg_mlp *m, float lr)
-{
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float b1 = 0.9f, b2 = 0.999f;
     m->step += 1;
-    const float bc1 = 1.0f - powf(b1, (float)m->step), bc2 = 1.0f - powf(b2, (float)m->step);
-    for (int l = 0; l < m->nl; ++l) {
-        hipLaunchKernelGGL(adam_kernel, dim3(2048), dim3(256), 0, m->st, m->dout[l], m->din[l], m->pin[l], m->W[l],
-                           m->G[l], m->Mw[l], m->Vw[l], lr, b1, b2, eps, bc1, bc2, m->W16[l]);
-        hipLaunchKernelGGL(adam_kernel, dim3(8), dim3(256), 0, m->st, 1, m->dout[l], m->pout[l], m->b[l], m->gb[l],
-                           m->Mb[l], m->Vb[l], lr, b1, b2, eps, bc1, bc2, (uint16_t *)nullptr);
-        MCHK(hipGetLastError());
-    }
+    return AdamArgs{lr, b1, b2, 1e-8f, 1.0f - powf(b1, (float)m->step), 1.0f - powf(b2, (float)m->step)};
+}
+
+static int read_loss(sfg_mlp *m, double *loss)
+{
+    double h = 0.0;
+    MCHK(hipMemcpyAsync(&h, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->st));
+    MCHK(hipStreamSynchronize(m->st));
+    *loss = h / ((double)m->batch * m->nout);
     return SFG_OK;
 }
 
@@ -608,9 +727,13 @@ extern "C" int sfg_mlp_set_data(sfg_mlp *m, const float *X, const float *Y, int 
 {
     g_mlp_err.clear();
     if (!m || !X || !Y || n <= 0) return mfail(SFG_E_ARG, "mlp data: bad arguments");
+    MCHK(hipStreamSynchronize(m->st));
     if (m->Xd) (void)hipFree(m->Xd);
     if (m->Yd) (void)hipFree(m->Yd);
+    if (m->order) (void)hipFree(m->order);
     m->Xd = m->Yd = nullptr;
+    m->order = nullptr;
+    MCHK(hipMalloc((void **)&m->order, sizeof(int) * (size_t)n));
     MCHK(hipMalloc((void **)&m->Xd, sizeof(float) * (size_t)n * m->nin));
     MCHK(hipMalloc((void **)&m->Yd, sizeof(float) * (size_t)n * m->nout));
     MCHK(hipMemcpy(m->Xd, X, sizeof(float) * (size_t)n * m->nin, hipMemcpyHostToDevice));
@@ -624,17 +747,16 @@ extern "C" int sfg_mlp_train_step(sfg_mlp *m, const int *idx, float lr, double *
 {
     g_mlp_err.clear();
     if (!m || !idx || !m->Xd) return mfail(SFG_E_ARG, "mlp train: no data");
-    int rc = load_batch(m, idx, m->Xd, m->Yd, m->ndata);
-    if (!rc) rc = forward(m);
-    if (!rc) rc = loss_grad(m, nullptr);
-    if (!rc) rc = backward(m);
-    if (!rc) rc = adam(m, lr);
-    if (!rc && loss) {
-        double h = 0.0;
-        MCHK(hipMemcpyAsync(&h, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->st));
-        MCHK(hipStreamSynchronize(m->st));
-        *loss = h / ((double)m->batch * m->nout);
+    int rc = upload_idx(m, idx, m->ndata);
+    if (!rc) rc = load_batch(m, m->idx, m->Xd);
+    if (!rc) rc = forward(m, m->Yd, m->idx);
+    if (!rc) {
+        const AdamArgs ad = next_adam(m, lr);
+        rc = backward(m, &ad);
     }
+    double h = 0.0;
+    if (!rc) rc = read_loss(m, &h); // also ends the host index buffer's use
+    if (!rc && loss) *loss = h;
     return rc;
 }
 
@@ -663,19 +785,23 @@ extern "C" int sfg_mlp_train(sfg_mlp *m, int epochs, float lr0, long long total_
             const int j = (int)((s * 2685821657736338717ull) % (uint64_t)(i + 1));
             std::swap(order[i], order[j]);
         }
+        // the epoch's order goes up once; the host copy is reused only after the
+        // stream has drained at the end of the epoch
+        MCHK(hipMemcpyAsync(m->order, order.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, m->st));
         for (int k = 0; k < per_epoch; ++k) {
             const double prog = std::min(1.0, (double)(m->step) / (double)T);
             const float lr = (float)(lr0 * 0.5 * (1.0 + std::cos(M_PI * prog)));
-            int rc = load_batch(m, order.data() + (size_t)k * B, m->Xd, m->Yd, n);
-            if (!rc) rc = forward(m);
-            if (!rc) rc = loss_grad(m, nullptr);
-            if (!rc) rc = backward(m);
-            if (!rc) rc = adam(m, lr);
+            const int *idx = m->order + (size_t)k * B;
+            int rc = load_batch(m, idx, m->Xd);
+            if (!rc) rc = forward(m, m->Yd, idx);
+            if (!rc) {
+                const AdamArgs ad = next_adam(m, lr);
+                rc = backward(m, &ad);
+            }
             if (rc) return rc;
-            // the index upload reads host memory asynchronously: keep it alive
-            MCHK(hipStreamSynchronize(m->st));
             ++done;
         }
+        MCHK(hipStreamSynchronize(m->st));
     }
     MCHK(hipEventRecord(e1, m->st));
     MCHK(hipEventSynchronize(e1));
@@ -710,7 +836,8 @@ extern "C" int sfg_mlp_predict(sfg_mlp *m, const float *X, int n, float *Y, doub
     for (int s0 = 0; s0 < n; s0 += B) {
         const int cnt = std::min(B, n - s0);
         for (int b = 0; b < B; ++b) idx[b] = s0 + std::min(b, cnt - 1);
-        int rc = load_batch(m, idx.data(), Xd, nullptr, n);
+        int rc = upload_idx(m, idx.data(), n);
+        if (!rc) rc = load_batch(m, m->idx, Xd);
         MCHK(hipEventRecord(e0, m->st));
         if (!rc) rc = forward(m);
         MCHK(hipEventRecord(e1, m->st));
@@ -744,10 +871,13 @@ extern "C" int sfg_mlp_gradients(sfg_mlp *m, const float *Xb, const float *Yb, d
     MCHK(hipMemcpy(Yd, Yb, sizeof(float) * (size_t)m->batch * m->nout, hipMemcpyHostToDevice));
     std::vector<int> idx(m->batch);
     for (int b = 0; b < m->batch; ++b) idx[b] = b;
-    int rc = load_batch(m, idx.data(), Xd, Yd, m->batch);
-    if (!rc) rc = forward(m);
-    if (!rc) rc = loss_grad(m, loss);
+    int rc = upload_idx(m, idx.data(), m->batch);
+    if (!rc) rc = load_batch(m, m->idx, Xd);
+    if (!rc) rc = forward(m, Yd, m->idx);
     if (!rc) rc = backward(m);
+    double h = 0.0;
+    if (!rc) rc = read_loss(m, &h);
+    if (!rc && loss) *loss = h;
     if (!rc) {
         std::vector<float> g((size_t)m->pout[layer] * m->pin[layer]), gb(m->pout[layer]);
         hipError_t e = hipStreamSynchronize(m->st);
@@ -803,8 +933,53 @@ extern "C" int sfg_gemm_bf16_nt(int M, int N, int K, const float *A, const float
     MCHK(hipMemcpy(db, bp.data(), sizeof(float) * bp.size(), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(256), dim3(256), 0, nullptr, ap.size(), da, ha);
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(256), dim3(256), 0, nullptr, bp.size(), db, hb);
-    MCHK(gemm(nullptr, M, N, Kp, ha, Kp, hb, Kp, dc, N, nullptr, EPI_STORE, nullptr, 0));
+    MCHK(gemm(nullptr, M, N, Kp, ha, Kp, hb, Kp, dc, N, EpiArgs{}));
     MCHK(hipMemcpy(C, dc, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToHost));
     for (void *p : {(void *)da, (void *)db, (void *)dc, (void *)ha, (void *)hb}) (void)hipFree(p);
+    return SFG_OK;
+}
+
+// pseudo-random bf16 in [-1, 1) (hash of the index)
+__global__ void fill_bf16_kernel(size_t n, uint32_t seed, uint16_t *p)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t h = (uint32_t)i * 2654435761u ^ seed;
+        h ^= h >> 15; h *= 2246822519u; h ^= h >> 13;
+        p[i] = f2bf((float)(h >> 8) * (2.0f / 16777216.0f) - 1.0f);
+    }
+}
+
+extern "C" int sfg_gemm_bf16_time(int M, int N, int K, int reps, double *ms)
+{
+    g_mlp_err.clear();
+    if (M <= 0 || N <= 0 || K <= 0 || K % 32 || reps <= 0 || !ms) return mfail(SFG_E_ARG, "gemm time: bad arguments");
+    uint16_t *a = nullptr, *b = nullptr;
+    float *c = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc((void **)&a, sizeof(uint16_t) * (size_t)M * K);
+    if (e == hipSuccess) e = hipMalloc((void **)&b, sizeof(uint16_t) * (size_t)N * K);
+    if (e == hipSuccess) e = hipMalloc((void **)&c, sizeof(float) * (size_t)M * N);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fill_bf16_kernel, dim3(1024), dim3(256), 0, st, (size_t)M * K, 1u, a);
+        hipLaunchKernelGGL(fill_bf16_kernel, dim3(1024), dim3(256), 0, st, (size_t)N * K, 2u, b);
+        e = gemm(st, M, N, K, a, K, b, K, c, N, EpiArgs{}); // warm-up
+    }
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = gemm(st, M, N, K, a, K, b, K, c, N, EpiArgs{});
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e == hipSuccess) *ms = (double)t / reps;
+    if (e1) (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (st) (void)hipStreamDestroy(st);
+    for (void *p : {(void *)a, (void *)b, (void *)c})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return mfail(SFG_E_HIP, "gemm time: %s", hipGetErrorString(e));
     return SFG_OK;
 }

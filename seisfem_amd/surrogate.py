@@ -118,6 +118,14 @@ def gemm_bf16(A, B):
     return Cm
 
 
+def gemm_bf16_time(M, N, K, reps=20):
+    """Average device time (ms) of one bf16 GEMM launch C[M][N] = A[M][K] . B[N][K]^T
+    on device-resident operands (K a multiple of 32)."""
+    ms = C.c_double(0.0)
+    _chk(lib().sfg_gemm_bf16_time(M, N, K, reps, C.byref(ms)), "sfg_gemm_bf16_time")
+    return ms.value
+
+
 # ---- layered models and their FEM shots -------------------------------------------
 
 def sample_models(n, seed, size=2000.0, src_depth=20.0):
