@@ -427,6 +427,16 @@ def main():
         grp.close()
         for r in runs:
             r.close()
+        # part 2 alone: the graph-replayed per-step launch against the persistent
+        # schedule (cooperative launch per guard interval, tile queues, grid barrier)
+        def part2_runs():
+            return [gpu.Run(keep[2 * qi], keep[2 * qi + 1], c["src"], L, c["rec"], c["dt"], c["f0"], c["t0"],
+                            precision=prec, nsteps=c["nsteps"], block=kw.get("block", 256), shots_per_lane_group=1,
+                            kernel_variant=8, xcd_remap=False) for qi, c in enumerate(models)]
+        best_s, tab_s = autotune.tune_multi_schedule(part2_runs, steps=min(300, L // 2), log=log)
+        multi["part2_schedule"] = {"ms_per_step": {("per_step" if k == 0 else f"persistent_{k}_per_cu"): ms
+                                                   for k, ms in tab_s},
+                                   "best": "per_step" if best_s == 0 else f"persistent_{best_s}_per_cu"}
         for h in keep:
             h.close()
 

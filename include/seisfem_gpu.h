@@ -232,6 +232,15 @@ int sfg_multi_create(sfg_multi **multi, sfg_run *const *runs, int nruns);
 /* Executes up to nsteps further steps of every run; returns the step count so far. */
 int sfg_multi_advance(sfg_multi *multi, int nsteps);
 double sfg_multi_last_ms_per_step(const sfg_multi *multi);
+/* Persistent schedule: blocks_per_cu > 0 replaces the per-step launches by one
+ * cooperative launch per guard interval, in which blocks_per_cu x (CU count)
+ * resident blocks take each step's tiles from a queue and meet at a grid-wide
+ * barrier between steps; 0 returns to the graph-replayed per-step launches.
+ * Results are identical bit for bit.  Needs every run in one shared launch
+ * (single-shot runs in the per-row or sliced layouts, or only batched runs
+ * too small to fill the GPU alone); fails if that many blocks cannot be
+ * resident at once.  No counterpart in the reference. */
+int sfg_multi_set_persistent(sfg_multi *multi, int blocks_per_cu);
 /* Algorithmic bytes of the launch of step `step` (shots still active then). */
 double sfg_multi_step_bytes(const sfg_multi *multi, int step);
 void sfg_multi_destroy(sfg_multi *multi);

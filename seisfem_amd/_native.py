@@ -93,6 +93,7 @@ def lib():
         "sfg_multi_create": (_i, [C.POINTER(_vp), C.POINTER(_vp), _i]),
         "sfg_multi_advance": (_i, [_vp, _i]),
         "sfg_multi_last_ms_per_step": (_d, [_vp]),
+        "sfg_multi_set_persistent": (_i, [_vp, _i]),
         "sfg_multi_step_bytes": (_d, [_vp, _i]),
         "sfg_multi_destroy": (None, [_vp]),
         "elastic_wave": (None, [C.c_char_p, _i, _i, _i, _pi, C.POINTER(_pd), _i, _i, _i, _d, _d, _d, _d,

@@ -133,3 +133,35 @@ def run_kwargs(best):
     d = dict(best)
     d.pop("row_order_name", None)
     return d
+
+
+def tune_multi_schedule(make_runs, steps=200, blocks_per_cu=(1, 2, 3), log=None):
+    """Schedule of a multi-run: graph-replayed per-step launches (0) or the
+    persistent cooperative launch with k resident blocks per CU (the knob).
+    make_runs() returns fresh gpu.Run objects of the job (at least 2 * steps
+    steps long); each candidate steps ``steps`` steps after as many untimed ones.  Returns (best k, table of
+    (k, ms/step)); a k whose blocks cannot all be resident is skipped."""
+    table = []
+    for k in (0,) + tuple(blocks_per_cu):
+        runs = make_runs()
+        multi = gpu.Multi(runs)
+        try:
+            multi.set_persistent(k)
+        except gpu.NativeError as e:
+            if e.code != -1:
+                raise
+            if log:
+                log(f"schedule {k} blocks/CU skipped: {e}")
+            k = None
+        if k is not None:
+            multi.advance(steps)
+            multi.advance(steps)
+            table.append((k, multi.last_ms_per_step))
+            if log:
+                log(f"schedule {'per-step launches' if k == 0 else f'persistent {k} blocks/CU'}: "
+                    f"{table[-1][1]:.4f} ms/step")
+        multi.close()
+        for r in runs:
+            r.close()
+    best = min(table, key=lambda c: c[1])[0]
+    return best, table

@@ -1339,6 +1339,11 @@ struct sfg_multi {
     std::vector<StepArgs<double>> solo64;   // their arguments with the shared counter
     std::vector<StepArgs<float>> solo32;
     MultiGroup grp[3];
+    // persistent schedule (sfg_multi_set_persistent): resident blocks of one
+    // cooperative launch per guard interval, or 0 for the per-step launches
+    int persist_blocks = 0;
+    int persist_group = -1;
+    int *queue = nullptr;       // device int[16]: tile queues by step parity and XCD
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // by parity of the first step
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1354,6 +1359,7 @@ static void multi_free(sfg_multi *m)
     if (m->ev1) (void)hipEventDestroy(m->ev1);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     dfree(m->ctrl);
+    dfree(m->queue);
     for (MultiGroup &g : m->grp) {
         dfree(g.args);
         dfree(g.tile_start);
@@ -1513,6 +1519,20 @@ extern "C" int sfg_multi_advance(sfg_multi *m, int nsteps)
     int todo = std::min(nsteps, m->max_steps - m->executed);
     const int steps = todo;
     HIPCHK(hipEventRecord(m->ev0, m->stream));
+    while (todo > 0 && m->persist_blocks > 0) {
+        // one cooperative launch up to the next guard
+        const MultiGroup &g = m->grp[m->persist_group];
+        const int seg = std::min(todo, m->guard_every - m->executed % m->guard_every);
+        HIPCHK(launch_multi_persistent(m->prec, g.args, g.tile_start, (int)g.runs.size(), g.ntiles, m->block,
+                                       m->persist_blocks, m->executed & 1, seg, m->queue, m->persist_group == 1,
+                                       m->stream));
+        m->executed += seg;
+        todo -= seg;
+        if (m->executed % m->guard_every == 0 || m->executed == m->max_steps) {
+            int rc = multi_launch_guards(m);
+            if (rc) return rc;
+        }
+    }
     while (todo > 0) {
         hipGraphExec_t g = m->graph[m->executed & 1];
         if (g && todo >= m->guard_every && m->executed % m->guard_every == 0) {
@@ -1541,6 +1561,40 @@ extern "C" int sfg_multi_advance(sfg_multi *m, int nsteps)
         if (rc) return rc;
     }
     return m->executed;
+}
+
+extern "C" int sfg_multi_set_persistent(sfg_multi *m, int blocks_per_cu)
+{
+    g_err.clear();
+    if (!m || blocks_per_cu < 0) return fail(SFG_E_ARG, "multi persistent: bad arguments");
+    if (blocks_per_cu == 0) {
+        m->persist_blocks = 0;
+        return SFG_OK;
+    }
+    int gi = -1, groups = 0;
+    for (int k = 0; k < 3; ++k)
+        if (!m->grp[k].runs.empty()) {
+            gi = k;
+            ++groups;
+        }
+    if (!m->solo.empty() || groups != 1 || gi == 2)
+        return fail(SFG_E_ARG, "multi persistent: needs every run in one shared launch (single-shot runs in the "
+                               "per-row or sliced layouts, or batched runs too small to fill the GPU alone)");
+    int per_cu = 0, dev = 0, cus = 0;
+    HIPCHK(multi_persistent_blocks_per_cu(m->prec, gi == 1, m->block, &per_cu));
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (blocks_per_cu > per_cu)
+        return fail(SFG_E_ARG, "multi persistent: %d blocks per CU requested, %d fit", blocks_per_cu, per_cu);
+    if (!m->queue) {
+        int rc = dalloc(&m->queue, 16);
+        if (rc) return rc;
+        HIPCHK(launch_fill_zero(m->queue, 16 * sizeof(int), m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+    }
+    m->persist_group = gi;
+    m->persist_blocks = blocks_per_cu * cus;
+    return SFG_OK;
 }
 
 extern "C" double sfg_multi_last_ms_per_step(const sfg_multi *m) { return m ? m->last_ms : 0.0; }

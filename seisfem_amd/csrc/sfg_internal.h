@@ -153,6 +153,12 @@ hipError_t allow_step_lds(int precision, int sp, bool coef_per_shot, size_t lds_
 // single_shot_only: every run has one shot per lane group (a kernel with only those forms)
 hipError_t launch_multi_step(int precision, const void *args_dev, const int *tile_start_dev, int nruns, int ntiles,
                              int block, int xcd_remap, int slot, bool single_shot_only, hipStream_t st);
+// Persistent schedule of a multi-run group (cooperative launch of `blocks`
+// resident blocks running nsteps steps; queue: device int[16], zero)
+hipError_t launch_multi_persistent(int precision, const void *args_dev, const int *tile_start_dev, int nruns,
+                                   int ntiles, int block, int blocks, int slot, int nsteps, int *queue,
+                                   bool single_shot_only, hipStream_t st);
+hipError_t multi_persistent_blocks_per_cu(int precision, bool single_shot_only, int block, int *per_cu);
 // multi-run launch of single-shot runs in the LDS-staged sliced layout (kernel variant 8)
 hipError_t launch_multi_step_lds(int precision, const void *args_dev, const int *tile_start_dev, int nruns,
                                  int ntiles, int block, int xcd_remap, int slot, size_t lds_bytes, hipStream_t st);

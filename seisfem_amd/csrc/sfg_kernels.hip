@@ -21,6 +21,8 @@
 //      (time_evolution/elastic_wave.c:582-592, 648-658, 681-729, 754-778);
 //   1  FP32 storage and arithmetic (FMA in the dot products);
 //   2  FP32 with a Kahan-compensated displacement update.
+#include <hip/hip_cooperative_groups.h>
+
 #include "sfg_internal.h"
 
 namespace sfg {
@@ -742,21 +744,13 @@ __global__ __launch_bounds__(256) void step_kernel(const StepArgs<T> a, int slot
 // of args[0].ctrl, and a run past its last step has no active shot.
 // SP1: only runs of one shot per lane group (their forms alone set the registers).
 template <class T, int PREC, bool SP1>
-__global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__restrict__ args,
-                                                         const int *__restrict__ tile_start, int nruns,
-                                                         int xcd_remap, int slot)
+__device__ __forceinline__ void multi_tile(const StepArgs<T> *__restrict__ args, const int *__restrict__ tile_start,
+                                           int nruns, int t, int bid)
 {
-    Ctrl *ctrl = args[0].ctrl;
-    const int t = begin_step(ctrl, slot);
-    const unsigned int nb = gridDim.x;
-    unsigned int bid = blockIdx.x;
-    // xcd_remap: 0 round-robin, 1 contiguous ranges, Q + 1 stripes of Q tiles
-    if (xcd_remap > 1) bid = xcd_striped(bid, nb, (unsigned int)(xcd_remap - 1));
-    else if (xcd_remap) bid = xcd_contiguous(bid, nb);
     int r = 0;
-    while (r + 1 < nruns && (int)bid >= tile_start[r + 1]) ++r;
+    while (r + 1 < nruns && bid >= tile_start[r + 1]) ++r;
     const StepArgs<T> &a = args[r];
-    const int local = (int)bid - tile_start[r];
+    const int local = bid - tile_start[r];
     const int bx = local % a.nbx, by = local / a.nbx;
     // Kernel variants of one launch: separate value arrays (0), packed values
     // (1; also runs 6), and for 32- and 64-shot groups streaming hints (2) and
@@ -790,6 +784,104 @@ __global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__re
     }
 #undef SFG_G
 #undef SFG_T
+}
+
+template <class T, int PREC, bool SP1>
+__global__ __launch_bounds__(256) void multi_step_kernel(const StepArgs<T> *__restrict__ args,
+                                                         const int *__restrict__ tile_start, int nruns,
+                                                         int xcd_remap, int slot)
+{
+    Ctrl *ctrl = args[0].ctrl;
+    const int t = begin_step(ctrl, slot);
+    const unsigned int nb = gridDim.x;
+    unsigned int bid = blockIdx.x;
+    // xcd_remap: 0 round-robin, 1 contiguous ranges, Q + 1 stripes of Q tiles
+    if (xcd_remap > 1) bid = xcd_striped(bid, nb, (unsigned int)(xcd_remap - 1));
+    else if (xcd_remap) bid = xcd_contiguous(bid, nb);
+    multi_tile<T, PREC, SP1>(args, tile_start, nruns, t, (int)bid);
+}
+
+// Persistent form of multi_step_kernel: a cooperative grid of resident blocks
+// runs nsteps steps in one launch, meeting at a grid-wide barrier between
+// steps (a step's gathers read what other blocks wrote in the previous one).
+// A step's tiles are split into 8 contiguous ranges, one per XCD, each with a
+// queue counter; a block takes PQ_CHUNK tiles at a time from the range of the
+// XCD it runs on (hardware register XCC_ID), then from the other ranges in
+// turn until all are exhausted.  Every tile is taken exactly once whatever XCD
+// a block reports, since each range's counter only grows.  queue[2][8]: the
+// counters by step parity; those of the next step are cleared during the
+// current one (nothing uses them after the previous barrier) and all are left
+// at zero.  The step counter in args[0].ctrl ends at t0 + nsteps in the slot
+// the next launch reads.
+constexpr int PQ_CHUNK = 4;
+
+template <class T, int PREC, bool SP1>
+__global__ __launch_bounds__(256, 2) void multi_persistent_kernel(const StepArgs<T> *__restrict__ args,
+                                                                  const int *__restrict__ tile_start, int nruns,
+                                                                  int ntiles, int slot, int nsteps, int *queue)
+{
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    Ctrl *ctrl = args[0].ctrl;
+    const int t0 = __hip_atomic_load(&ctrl->step[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    // HW_REG_XCC_ID (id 20), bits 3:0
+    const int xcc = (int)(__builtin_amdgcn_s_getreg(20 | (3 << 11)) & 7u);
+    __shared__ int tile_s;
+    for (int s = 0; s < nsteps; ++s) {
+        int *q = queue + 8 * (s & 1);
+        if (lead)
+            for (int k = 0; k < 8; ++k)
+                __hip_atomic_store(queue + 8 * ((s + 1) & 1) + k, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < 8; ++k) {
+            const int x = (xcc + k) & 7;
+            const int lo = (int)((long long)ntiles * x / 8), hi = (int)((long long)ntiles * (x + 1) / 8);
+            for (;;) {
+                if (threadIdx.x == 0)
+                    tile_s = lo + PQ_CHUNK * __hip_atomic_fetch_add(q + x, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __syncthreads();
+                const int first = tile_s;
+                __syncthreads();
+                if (first >= hi) break;
+                for (int tile = first; tile < min(hi, first + PQ_CHUNK); ++tile)
+                    multi_tile<T, PREC, SP1>(args, tile_start, nruns, t0 + s, tile);
+            }
+        }
+        grid.sync();
+    }
+    if (lead) {
+        __hip_atomic_store(&ctrl->step[slot ^ (nsteps & 1)], t0 + nsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < 16; ++k) __hip_atomic_store(queue + k, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <bool SP1>
+static const void *persistent_fn(int precision)
+{
+    switch (precision) {
+    case 0: return (const void *)multi_persistent_kernel<double, 0, SP1>;
+    case 1: return (const void *)multi_persistent_kernel<float, 1, SP1>;
+    case 2: return (const void *)multi_persistent_kernel<float, 2, SP1>;
+    case 3: return (const void *)multi_persistent_kernel<float, 3, SP1>;
+    default: return nullptr;
+    }
+}
+
+hipError_t multi_persistent_blocks_per_cu(int precision, bool single_shot_only, int block, int *per_cu)
+{
+    const void *f = single_shot_only ? persistent_fn<true>(precision) : persistent_fn<false>(precision);
+    if (!f) return hipErrorInvalidValue;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, f, block, 0);
+}
+
+hipError_t launch_multi_persistent(int precision, const void *args_dev, const int *tile_start_dev, int nruns,
+                                   int ntiles, int block, int blocks, int slot, int nsteps, int *queue,
+                                   bool single_shot_only, hipStream_t st)
+{
+    const void *f = single_shot_only ? persistent_fn<true>(precision) : persistent_fn<false>(precision);
+    if (!f) return hipErrorInvalidValue;
+    void *kargs[] = {(void *)&args_dev, (void *)&tile_start_dev, (void *)&nruns, (void *)&ntiles,
+                     (void *)&slot, (void *)&nsteps, (void *)&queue};
+    return hipLaunchCooperativeKernel(f, dim3(blocks), dim3(block), kargs, 0, st);
 }
 
 // The single-shot runs of a multi-run in the LDS-staged sliced layout (kernel

@@ -206,6 +206,12 @@ class Multi:
     def last_ms_per_step(self):
         return lib().sfg_multi_last_ms_per_step(self.h)
 
+    def set_persistent(self, blocks_per_cu):
+        """blocks_per_cu > 0: one cooperative launch per guard interval with that
+        many resident blocks per CU pulling each step's tiles from a queue; 0: the
+        graph-replayed per-step launches (default)."""
+        check(lib().sfg_multi_set_persistent(self.h, int(blocks_per_cu)), "sfg_multi_set_persistent")
+
     def step_bytes(self, step):
         return lib().sfg_multi_step_bytes(self.h, int(step))
 

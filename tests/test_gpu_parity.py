@@ -632,6 +632,83 @@ def test_multi_run_one_launch_per_step_matches_separate_runs():
         r.close()
 
 
+@pytest.mark.parametrize("prec,kind", [(gpu.FP64, "single"), (gpu.FP32_KAHAN16, "single"), (gpu.FP64, "batched"),
+                                       (gpu.FP32_KAHAN, "batched")])
+def test_multi_run_persistent_schedule_matches_per_step_launches(prec, kind):
+    """The persistent schedule (one cooperative launch per guard interval,
+    resident blocks taking each step's tiles from a queue, a grid barrier
+    between steps) gives the per-step launches' seismograms bit for bit: with
+    1 and 2 blocks per CU, advanced in pieces that end inside guard intervals,
+    and switched on part-way through a run.  It refuses groups that need more
+    than one launch per step and more blocks than fit."""
+    if kind == "single":
+        specs = [(host.Q4, 10, 1, 90, 0, 1), (host.Q4, 13, 1, 55, 0, 1), (host.Q9, 8, 1, 90, 0, 8),
+                 (host.T6, 9, 1, 70, 0, 0)]
+    else:
+        specs = [(host.Q4, 12, 70, 90, 64, 1), (host.Q4, 11, 45, 90, 32, 5)]
+    say(f"multi-run persistent schedule vs per-step launches: {kind}, precision {prec}")
+    keep, sets = [], []
+    for tc, nelem, nshots, steps, sp, variant in specs:
+        case = _case(tc, nelem, 90, nshots=nshots)
+        prob = case["problem"]
+        op = gpu.Operator(prob["node_num"], prob["csr"])
+        md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
+        keep += [op, md]
+        nsteps = np.full(nshots, steps, np.int32)
+        if nshots > 1:
+            nsteps = np.random.default_rng(2).integers(30, 91, nshots).astype(np.int32)
+            nsteps[0] = 90
+        sets.append(lambda op=op, md=md, case=case, nsteps=nsteps, sp=sp, variant=variant: gpu.Run(
+            op, md, case["src"], 90, case["rec"], case["dt"], case["f0"], case["t0"], precision=prec,
+            nsteps=nsteps, guard_every=25, block=256, shots_per_lane_group=sp, kernel_variant=variant))
+
+    def job(schedule):
+        runs = [mk() for mk in sets]
+        multi = gpu.Multi(runs)
+        for bpc, n in schedule:
+            multi.set_persistent(bpc)
+            multi.advance(n)
+        out = [r.seismograms() for r in runs]
+        multi.close()
+        for r in runs:
+            r.close()
+        return out
+
+    ref = job([(0, 90)])
+    assert max(np.max(np.abs(u)) for u, _ in ref) > 0
+    for schedule in ([(1, 37), (1, 53)], [(2, 90)], [(0, 30), (1, 60)]):
+        got = job(schedule)
+        for (ua, wa), (ub, wb) in zip(got, ref):
+            assert bits_equal(ua, ub) and bits_equal(wa, wb), schedule
+    # refusals: a group with a run of its own launch (variant 9) and too many blocks per CU
+    runs = [mk() for mk in sets]
+    multi = gpu.Multi(runs)
+    with pytest.raises(gpu.NativeError) as ei:
+        multi.set_persistent(1000)
+    assert ei.value.code == -1
+    multi.close()
+    for r in runs:
+        r.close()
+    if kind == "single":
+        case = _case(host.Q4, 17, 90)
+        p = case["problem"]
+        op = gpu.Operator(p["node_num"], p["csr"])
+        md = gpu.Medium(op, p["c"], p["mass_lump"], p["dirichlet"], p["mpml"])
+        keep += [op, md]
+        runs = [sets[0](), gpu.Run(op, md, case["src"], 90, case["rec"], case["dt"], case["f0"], case["t0"],
+                                   precision=prec, guard_every=25, block=256, kernel_variant=9,
+                                   row_order=host.patch_order(p["x"], p["y"], 16, 16))]
+        multi = gpu.Multi(runs)
+        with pytest.raises(gpu.NativeError) as ei:
+            multi.set_persistent(1)
+        assert ei.value.code == -1
+        multi.close()
+        for r in runs:
+            r.close()
+    for h in keep:
+        h.close()
+
+
 def test_multi_run_fp32_forms_match_separate_runs():
     """The same four runs in the FP32 form with bfloat16 Kahan terms: one
     launch per step for all of them gives the seismograms each run gives
