@@ -453,6 +453,10 @@ def main():
                "omp": f"OMP_PROC_BIND={cb['omp_proc_bind']} OMP_PLACES={cb['omp_places']}",
                "with_text_io": {"value": cb["dof_steps_per_s_with_io"], "ms_per_step": cb["ms_per_step_with_io"],
                                 "bytes_written": cb["io_bytes"]},
+               "reference_form": {"value": cb["dof_steps_per_s_reference_form"],
+                                  "ms_per_step": cb["ms_per_step_reference_form"],
+                                  "what": "with the text output and the reference's shared, unreduced energy "
+                                          "sums in the parallel update loop (elastic_wave.c:776-784)"},
                "ms_per_step": cb["ms_per_step_no_io"],
                "sample": f"FP64, one shot of the same Q4 {args.nelem}x{args.nelem} mesh and model, first "
                          f"{cb['steps']} steps; step loop alone timed with omp_get_wtime (arrays allocated and "

@@ -55,6 +55,8 @@ def olib():
         L.oracle_elastic_masslump.restype = _i
         L.oracle_elastic_masslump_timed.argtypes = L.oracle_elastic_masslump.argtypes + [C.c_char_p, _pd]
         L.oracle_elastic_masslump_timed.restype = _i
+        L.oracle_set_shared_energy.argtypes = [_i]
+        L.oracle_set_shared_energy.restype = None
         _olib = L
     return _olib
 
@@ -210,6 +212,13 @@ def oracle_csr_matvec(p, j, x_, v):
                              _dp(np.ascontiguousarray(x_, np.float64)), _dp(np.ascontiguousarray(v, np.float64)),
                              _dp(y))
     return y
+
+
+def set_shared_energy(on):
+    """Timing only: the loop's energy sums accumulated the reference's way (one
+    shared pair of sums written by every thread, elastic_wave.c:776-784) instead
+    of by a reduction.  The seismograms are unchanged; the sums are not exact."""
+    olib().oracle_set_shared_energy(1 if on else 0)
 
 
 def oracle_masslump(prob, src_node, step, rec_nodes, dt, f0, t0, amplitude=1.0e10, angle_deg=90.0,

@@ -4,9 +4,10 @@ Times the C restatement of the reference's masslump loop (sfem_oracle.c,
 time_evolution/elastic_wave.c:446-809 with the 44 csr_matvec calls per step of
 :533-580 and :599-646) on the host cores, FP64, one shot of the benchmark's
 mesh, as the reference runs it.  The step loop alone is timed with
-omp_get_wtime(), once without output and once with the reference's %f text
+omp_get_wtime(), once without output, once with the reference's %f text
 output (seismogram rows every step, wavefield rows every 200 steps,
-:787-806) written to a temporary directory.
+:787-806) written to a temporary directory, and once more with that output
+and the reference's own shared (unreduced) energy accumulation, :776-784.
 
 Run by bench.py in a child process that touches no GPU, so that the OpenMP
 runtime starts with OMP_NUM_THREADS = the physical cores it may use,
@@ -80,13 +81,23 @@ def main():
         assert r["status"] == k, r["status"]
         return r["loop_seconds"]
 
-    # calibration, then two loops of about half the budget each (whole shots when they fit)
+    # calibration, then three loops of about a third of the budget each (whole shots when
+    # they fit): without output, with the %f text output, and with the output and the
+    # reference's own energy accumulation (elastic_wave.c:776-784: every thread adds into one
+    # shared sum inside the parallel update loop, no reduction; its cache line moves between
+    # the cores at every node, which dominates the reference's time at several threads)
     per_step = loop(10) / 10.0
-    k = int(min(args.max_steps, max(200, 0.5 * args.seconds / max(per_step, 1e-9))))
+    k = int(min(args.max_steps, max(200, args.seconds / 3.0 / max(per_step, 1e-9))))
     t_noio = loop(k)
     with tempfile.TemporaryDirectory() as d:
         t_io = loop(k, io_dir=d)
         io_bytes = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+    oracle.set_shared_energy(True)
+    try:
+        with tempfile.TemporaryDirectory() as d:
+            t_ref = loop(k, io_dir=d)
+    finally:
+        oracle.set_shared_energy(False)
     n = prob["node_num"]
     out = {
         "threads": int(os.environ.get("OMP_NUM_THREADS", "0") or 0),
@@ -96,6 +107,8 @@ def main():
         "loop_s_no_io": t_noio, "loop_s_with_io": t_io, "io_bytes": io_bytes,
         "dof_steps_per_s_no_io": 2.0 * n * k / t_noio, "dof_steps_per_s_with_io": 2.0 * n * k / t_io,
         "ms_per_step_no_io": 1e3 * t_noio / k, "ms_per_step_with_io": 1e3 * t_io / k, "setup_s": setup_s,
+        "loop_s_reference_form": t_ref, "ms_per_step_reference_form": 1e3 * t_ref / k,
+        "dof_steps_per_s_reference_form": 2.0 * n * k / t_ref,
     }
     print(json.dumps(out), flush=True)
 

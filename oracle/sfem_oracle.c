@@ -43,6 +43,11 @@
 #define ORACLE_E_ENERGY (-6)
 #define ORACLE_E_NOMEM (-4)
 
+/* 1: the timed loop accumulates the energy sums the reference's way (see the
+   Newmark update below); for the CPU-baseline timing only */
+static int g_shared_energy = 0;
+void oracle_set_shared_energy(int on) { g_shared_energy = on != 0; }
+
 void oracle_csr_matvec(int Bp_size, const int *Bp, const int *Bj, const double *Bx,
                        const double *x, double *y)
 {
@@ -343,27 +348,59 @@ int oracle_elastic_masslump_timed(int node_num, const int *const csr_p[7], const
             }
         }
         /* Newmark update (alpha = 1, delta = 1.5) + recombination, :754-786 */
+#define NEWMARK_ROW(i)                                                                                  \
+    do {                                                                                                \
+        U1[i] = U1[i] + U1t[i] * dt + ((0.5 - alpha) * U1tt[i] + alpha * U1n[i]) * dt * dt;             \
+        U2[i] = U2[i] + U2t[i] * dt + ((0.5 - alpha) * U2tt[i] + alpha * U2n[i]) * dt * dt;             \
+        U3[i] = U3[i] + U3t[i] * dt + ((0.5 - alpha) * U3tt[i] + alpha * U3n[i]) * dt * dt;             \
+        W1[i] = W1[i] + W1t[i] * dt + ((0.5 - alpha) * W1tt[i] + alpha * W1n[i]) * dt * dt;             \
+        W2[i] = W2[i] + W2t[i] * dt + ((0.5 - alpha) * W2tt[i] + alpha * W2n[i]) * dt * dt;             \
+        W3[i] = W3[i] + W3t[i] * dt + ((0.5 - alpha) * W3tt[i] + alpha * W3n[i]) * dt * dt;             \
+        U1t[i] = U1t[i] + ((1 - delta) * U1tt[i] + delta * U1n[i]) * dt;                               \
+        U2t[i] = U2t[i] + ((1 - delta) * U2tt[i] + delta * U2n[i]) * dt;                               \
+        U3t[i] = U3t[i] + ((1 - delta) * U3tt[i] + delta * U3n[i]) * dt;                               \
+        W1t[i] = W1t[i] + ((1 - delta) * W1tt[i] + delta * W1n[i]) * dt;                               \
+        W2t[i] = W2t[i] + ((1 - delta) * W2tt[i] + delta * W2n[i]) * dt;                               \
+        W3t[i] = W3t[i] + ((1 - delta) * W3tt[i] + delta * W3n[i]) * dt;                               \
+        U1tt[i] = U1n[i]; U2tt[i] = U2n[i]; U3tt[i] = U3n[i];                                           \
+        W1tt[i] = W1n[i]; W2tt[i] = W2n[i]; W3tt[i] = W3n[i];                                           \
+        U[i] = U1[i] + U2[i] + U3[i];                                                                   \
+        W[i] = W1[i] + W2[i] + W3[i];                                                                   \
+    } while (0)
+        if (!g_shared_energy) {
 #pragma omp parallel for private(i) reduction(+ : Eu, Ew)
-        for (i = 0; i < N; i++) {
-            U1[i] = U1[i] + U1t[i] * dt + ((0.5 - alpha) * U1tt[i] + alpha * U1n[i]) * dt * dt;
-            U2[i] = U2[i] + U2t[i] * dt + ((0.5 - alpha) * U2tt[i] + alpha * U2n[i]) * dt * dt;
-            U3[i] = U3[i] + U3t[i] * dt + ((0.5 - alpha) * U3tt[i] + alpha * U3n[i]) * dt * dt;
-            W1[i] = W1[i] + W1t[i] * dt + ((0.5 - alpha) * W1tt[i] + alpha * W1n[i]) * dt * dt;
-            W2[i] = W2[i] + W2t[i] * dt + ((0.5 - alpha) * W2tt[i] + alpha * W2n[i]) * dt * dt;
-            W3[i] = W3[i] + W3t[i] * dt + ((0.5 - alpha) * W3tt[i] + alpha * W3n[i]) * dt * dt;
-            U1t[i] = U1t[i] + ((1 - delta) * U1tt[i] + delta * U1n[i]) * dt;
-            U2t[i] = U2t[i] + ((1 - delta) * U2tt[i] + delta * U2n[i]) * dt;
-            U3t[i] = U3t[i] + ((1 - delta) * U3tt[i] + delta * U3n[i]) * dt;
-            W1t[i] = W1t[i] + ((1 - delta) * W1tt[i] + delta * W1n[i]) * dt;
-            W2t[i] = W2t[i] + ((1 - delta) * W2tt[i] + delta * W2n[i]) * dt;
-            W3t[i] = W3t[i] + ((1 - delta) * W3tt[i] + delta * W3n[i]) * dt;
-            U1tt[i] = U1n[i]; U2tt[i] = U2n[i]; U3tt[i] = U3n[i];
-            W1tt[i] = W1n[i]; W2tt[i] = W2n[i]; W3tt[i] = W3n[i];
-            U[i] = U1[i] + U2[i] + U3[i];
-            W[i] = W1[i] + W2[i] + W3[i];
-            Eu += U[i] * U[i];
-            Ew += W[i] * W[i];
+            for (i = 0; i < N; i++) {
+                NEWMARK_ROW(i);
+                Eu += U[i] * U[i];
+                Ew += W[i] * W[i];
+            }
+        } else {
+            /* The reference's own form (:776-784): every thread adds each node's U^2, W^2
+               into one shared pair of sums inside the parallel loop (no reduction) and tests
+               the guard there.  Relaxed atomic loads and stores give the same memory traffic
+               (the sums' cache line moving between cores) without a data race; updates
+               that cross are lost, as in the reference, so these sums only time the form. */
+            /* the two sums on lines of their own, as the reference's Energy_u / Energy_w arrays */
+            double es_u[8] __attribute__((aligned(64))) = {0.0}, es_w[8] __attribute__((aligned(64))) = {0.0};
+            double *es[2] = {es_u, es_w};
+            int hit = 0;
+#pragma omp parallel for private(i)
+            for (i = 0; i < N; i++) {
+                NEWMARK_ROW(i);
+                double eu, ew;
+                __atomic_load(es[0], &eu, __ATOMIC_RELAXED);
+                eu += U[i] * U[i];
+                __atomic_store(es[0], &eu, __ATOMIC_RELAXED);
+                __atomic_load(es[1], &ew, __ATOMIC_RELAXED);
+                ew += W[i] * W[i];
+                __atomic_store(es[1], &ew, __ATOMIC_RELAXED);
+                if (eu > 10e6 || ew > 10e6) __atomic_store_n(&hit, 1, __ATOMIC_RELAXED);
+            }
+            Eu = es_u[0];
+            Ew = es_w[0];
+            if (hit) Eu = 2.0 * 10e6;
         }
+#undef NEWMARK_ROW
         if (energy) {
             energy[2 * it] = Eu;
             energy[2 * it + 1] = Ew;
