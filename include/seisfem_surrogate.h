@@ -39,6 +39,8 @@ int sfg_mlp_gradients(sfg_mlp *m, const float *Xb, const float *Yb, double *loss
                       float *db);
 /* FP32 master weights of one layer: W [dout][din], b [dout]. */
 int sfg_mlp_get_layer(sfg_mlp *m, int layer, float *W, float *b);
+/* Sets the FP32 master weights of one layer (and their bf16 copy): W [dout][din], b [dout]. */
+int sfg_mlp_set_layer(sfg_mlp *m, int layer, const float *W, const float *b);
 /* C [M][N] = bf16(A) [M][K] . bf16(B)^T [K][N] with FP32 accumulation (kernel test entry). */
 int sfg_gemm_bf16_nt(int M, int N, int K, const float *A, const float *B, float *C);
 /* Times `reps` launches of the same GEMM on device-resident bf16 operands with

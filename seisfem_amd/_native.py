@@ -112,6 +112,7 @@ def lib():
         "sfg_mlp_predict": (_i, [_vp, _pf, _i, _pf, _pd]),
         "sfg_mlp_gradients": (_i, [_vp, _pf, _pf, _pd, _i, _pf, _pf]),
         "sfg_mlp_get_layer": (_i, [_vp, _i, _pf, _pf]),
+        "sfg_mlp_set_layer": (_i, [_vp, _i, _pf, _pf]),
         "sfg_gemm_bf16_nt": (_i, [_i, _i, _i, _pf, _pf, _pf]),
         "sfg_gemm_bf16_time": (_i, [_i, _i, _i, _i, C.POINTER(C.c_double)]),
     }

@@ -912,6 +912,27 @@ extern "C" int sfg_mlp_get_layer(sfg_mlp *m, int layer, float *W, float *b)
     return SFG_OK;
 }
 
+// W [dout][din], b [dout] (host) -> master weights of layer l and their bf16 copy
+// (the Adam moments are left as they are)
+extern "C" int sfg_mlp_set_layer(sfg_mlp *m, int layer, const float *W, const float *b)
+{
+    g_mlp_err.clear();
+    if (!m || layer < 0 || layer >= m->nl || !W || !b) return mfail(SFG_E_ARG, "mlp set: bad arguments");
+    std::vector<float> g((size_t)m->pout[layer] * m->pin[layer], 0.0f), gb(m->pout[layer], 0.0f);
+    for (int r = 0; r < m->dout[layer]; ++r) {
+        std::memcpy(g.data() + (size_t)r * m->pin[layer], W + (size_t)r * m->din[layer],
+                    sizeof(float) * m->din[layer]);
+        gb[r] = b[r];
+    }
+    MCHK(hipStreamSynchronize(m->st));
+    MCHK(hipMemcpy(m->W[layer], g.data(), sizeof(float) * g.size(), hipMemcpyHostToDevice));
+    MCHK(hipMemcpy(m->b[layer], gb.data(), sizeof(float) * gb.size(), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(1024), dim3(256), 0, m->st, g.size(), m->W[layer], m->W16[layer]);
+    MCHK(hipGetLastError());
+    MCHK(hipStreamSynchronize(m->st));
+    return SFG_OK;
+}
+
 // Plain bf16 GEMM C = A . B^T on host arrays (A [M][K], B [N][K] as fp32, rounded
 // to bf16 here; K padded to 32 inside).  For kernel tests.
 extern "C" int sfg_gemm_bf16_nt(int M, int N, int K, const float *A, const float *B, float *C)

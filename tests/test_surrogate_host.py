@@ -36,3 +36,29 @@ def test_sampled_models_are_ordered_and_normalised():
     assert m["X"].shape == (50, surrogate.NIN)
     assert np.all(np.diff(m["depths"], axis=1) >= 0) and np.all(np.diff(m["vp"], axis=1) >= 0)
     assert np.all(m["X"] >= 0) and np.all(m["X"] <= 1.0)
+
+
+def test_error_gate_estimates_and_calibrates_on_synthetic_errors():
+    """The gate's regression recovers a known relation between the spread and
+    the error, and its threshold keeps the accepted validation models at the
+    requested precision (checked on fresh draws of the same relation)."""
+    rng = np.random.default_rng(3)
+
+    def draw(n):
+        X = rng.uniform(0.0, 1.0, (n, 13))
+        X[:, 12] = 0.01  # a constant input (the source depth) is dropped
+        spread = np.exp(rng.uniform(np.log(0.005), np.log(0.2), n))
+        err = 1.5 * spread * np.exp(0.3 * X[:, 0] + rng.normal(0.0, 0.15, n))
+        return X, spread, err
+
+    X, s, e = draw(4000)
+    gate = surrogate.ErrorGate(bar=0.05, precision=0.9).fit(X, s, e)
+    Xt, st, et = draw(4000)
+    est = gate.estimate(Xt, st)
+    assert np.corrcoef(np.log(est), np.log(et))[0, 1] > 0.9
+    acc = gate.accept(Xt, st)
+    assert 0.1 < acc.mean() < 0.9
+    assert (et[acc] < 0.05).mean() >= 0.85
+    # nothing below the bar: nothing accepted
+    gate0 = surrogate.ErrorGate(bar=0.05).fit(X, s, e + 1.0)
+    assert not gate0.accept(Xt, st).any()
