@@ -327,6 +327,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         // nonzeros k0 .. k1-1 of the row, or slots k0 .. k1-1 of its slice
         const int k0 = SELL ? ro(a.sell_off)[slice] : ro(a.rp)[row];
         const int k1 = SELL ? ro(a.sell_off)[slice + 1] : ro(a.rp)[row + 1];
+        // sliced layout: slots of a lane past its own row's length are padding
+        // (column = the row itself, or its LDS slot; values 0); they are handed
+        // over as such without loading them, so a cache line of the slot arrays
+        // that only padding lanes would touch is not fetched (a slice's rows are
+        // sorted by length, so those lanes are contiguous)
+        const int rlen = SELL ? ro(a.rp)[row + 1] - ro(a.rp)[row] : 0;
+        const int pad_j = LDS1 ? ro(a.sell_own)[(size_t)slice * 64 + lane] : row;
         const size_t o = node_base(row);
         T cur[NFIELD];
         T own_u = 0, own_w = 0;                  // summed rows: the row's own U, W
@@ -450,10 +457,16 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
         auto load_slot = [&](int k, auto nv) __attribute__((always_inline)) {
             constexpr int NV = decltype(nv)::value;
             Slot q;
-            q.j = ld_once<true>(a.sell_ci + (size_t)k * 64 + lane);
-            const T *v = a.sell_val + (size_t)k * (7 * 64) + lane;
+            if (k - k0 < rlen) {
+                q.j = ld_once<true>(a.sell_ci + (size_t)k * 64 + lane);
+                const T *v = a.sell_val + (size_t)k * (7 * 64) + lane;
 #pragma unroll
-            for (int m = 0; m < 7; ++m) q.v[m] = m < NV ? ld_once<true>(v + 64 * m) : T(0);
+                for (int m = 0; m < 7; ++m) q.v[m] = m < NV ? ld_once<true>(v + 64 * m) : T(0);
+            } else {
+                q.j = pad_j;
+#pragma unroll
+                for (int m = 0; m < 7; ++m) q.v[m] = T(0);
+            }
             return q;
         };
         auto walk = [&](auto &&nz, auto per_trip, auto nv) __attribute__((always_inline)) {
@@ -462,8 +475,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<T> &a, int t, int bx, i
                 // the slots' loads run SELL_AHEAD slots ahead of their use (a ring of
                 // registers), so a wave keeps several slots' loads in flight.  Every
                 // lane walks to the slice's longest row (its padded slots add exact
-                // zeros): bounding each lane by its own row measured slower, the
-                // divergent ring costing more than the padded lines it skips.
+                // zeros, loaded or not): stopping each lane at its own row measured
+                // slower, the divergent ring costing more than the lines it skips.
                 constexpr int PA = SELL_AHEAD;
                 Slot ring[PA];
 #pragma unroll
