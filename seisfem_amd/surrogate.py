@@ -266,16 +266,23 @@ def rel_l2_rows(A, B):
     return np.linalg.norm(A - B, axis=1) / np.maximum(np.linalg.norm(B, axis=1), 1e-30)
 
 
-def train_member(X, Y, nfit, hidden, batch, seed, epochs, lr=1e-3, check_every=100, patience=2, log=None):
+def train_member(X, Y, nfit, hidden, batch, seed, epochs, lr=1e-3, check_every=100, patience=2, log=None,
+                 mirror_nrec=0):
     """One network of the ensemble, trained on the first nfit models with Adam and a
     cosine-decayed learning rate over `epochs`.  Every `check_every` epochs the
     median rel-L2 on the validation models (X[nfit:], Y[nfit:]) is measured; the
     weights of the best check are kept, training stops after `patience` checks
     without improvement, and the best weights are restored if the last check was
-    worse.  Returns (net, best validation median, ms per Adam step)."""
+    worse.  With mirror_nrec > 0 the training set also holds the mirror image of
+    each training model (``mirrored``, mirror_nrec receivers).  Returns (net, best
+    validation median, ms per Adam step)."""
     net = MLP(NIN, hidden, 3, Y.shape[1], batch=batch, seed=seed)
-    net.set_data(X[:nfit], Y[:nfit])
-    total = (nfit // batch) * epochs
+    Xf, Yf = X[:nfit], Y[:nfit]
+    if mirror_nrec:
+        Xm, Ym = mirrored(Xf, Yf, mirror_nrec)
+        Xf, Yf = np.concatenate([Xf, Xm]), np.concatenate([Yf, Ym])
+    net.set_data(Xf, Yf)
+    total = (Xf.shape[0] // batch) * epochs
     best, best_w, worse, done, ms_sum, nchunks, val = float("inf"), None, 0, 0, 0.0, 0, float("inf")
     while done < epochs:
         c = min(check_every, epochs - done)
@@ -295,6 +302,17 @@ def train_member(X, Y, nfit, hidden, batch, seed, epochs, lr=1e-3, check_every=1
     if best_w is not None and val > best:
         net.set_weights(best_w)
     return net, best, ms_sum / max(nchunks, 1)
+
+
+def mirrored(X, Y, nrec, size_col=11):
+    """Training pairs of the mirror image of each model: the layered model and the
+    mesh, its M-PML layers and the receiver line are symmetric about the centre of
+    the line, so the shot of a source at x (input column size_col holds x / size)
+    mirrored is the shot of a source at size - x with the receiver order reversed."""
+    Xm = np.array(X, copy=True)
+    Xm[:, size_col] = 1.0 - Xm[:, size_col]
+    Ym = np.ascontiguousarray(Y.reshape(Y.shape[0], nrec, -1)[:, ::-1, :].reshape(Y.shape))
+    return Xm, Ym
 
 
 def ensemble_predict(nets, X):
@@ -409,6 +427,7 @@ def main():
     ap.add_argument("--threshold", type=float, default=0.05)
     ap.add_argument("--precision", type=float, default=0.9, help="gate precision on the validation models")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-mirror", action="store_true", help="train without the mirrored models")
     args = ap.parse_args()
 
     def log(m):
@@ -428,7 +447,8 @@ def main():
     nets, vals, mss = [], [], []
     for k in range(args.ensemble):
         net, val, ms = train_member(X[:args.train], Y[:args.train], nfit, args.hidden, args.batch,
-                                    args.seed + 1 + k, args.epochs, log=log)
+                                    args.seed + 1 + k, args.epochs, log=log,
+                                    mirror_nrec=0 if args.no_mirror else gen.rec.size)
         nets.append(net)
         vals.append(val)
         mss.append(ms)
@@ -441,7 +461,7 @@ def main():
         f"gain {rep['gain']:.4f}")
     out = {"config": f"surrogate 13-{args.hidden}x3-{Y.shape[1]}, {args.train} FEM shots Q4 {args.nelem}x"
                      f"{args.nelem}, {args.steps} steps, Ricker {args.f0} Hz", "params": nets[0].params,
-           "ensemble": args.ensemble, "epochs": args.epochs, "train_ms_per_step": float(np.mean(mss)),
+           "ensemble": args.ensemble, "epochs": args.epochs, "mirrored_training_models": not args.no_mirror, "train_ms_per_step": float(np.mean(mss)),
            "train_wall_s": train_s, "validation_median_rel_l2": vals, "fem_device_s": fem_s, **rep}
     print(json.dumps(out), flush=True)
     for net in nets:

@@ -91,6 +91,23 @@ def test_training_reduces_loss():
     net.close()
 
 
+def test_mirrored_shots_are_mirrored_traces():
+    """The FEM shot of a layered model with the source at size - x equals the shot
+    with the source at x with the receiver order reversed (the symmetry the
+    surrogate's mirrored training models rely on), to FP32 rounding."""
+    gen = surrogate.FEMGenerator(nelem=60, edge=2000.0 / 60, pml=8, dt=1e-3, steps=1000, nrec=32, decim=4, f0=3.0)
+    m = surrogate.sample_models(8, seed=21, size=gen.L)
+    mm = dict(m, src_x=gen.L - m["src_x"])
+    say("mirror symmetry: 8 models Q4 60x60, 1000 steps")
+    Y, _ = gen.generate(m, batch=8)
+    Ym, _ = gen.generate(mm, batch=8)
+    gen.close()
+    _, Yr = surrogate.mirrored(m["X"], Y, 32)
+    err = surrogate.rel_l2_rows(Ym, Yr)
+    say(f"  max rel-L2 {err.max():.2e}")
+    assert err.max() < 1e-4
+
+
 def test_surrogate_accepts_held_out_models_at_five_percent():
     """FEM-generated data -> an ensemble of three GPU-trained networks -> the
     error gate (no FEM trace) -> held-out models: some are accepted at the 5 %

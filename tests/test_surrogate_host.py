@@ -62,3 +62,22 @@ def test_error_gate_estimates_and_calibrates_on_synthetic_errors():
     # nothing below the bar: nothing accepted
     gate0 = surrogate.ErrorGate(bar=0.05).fit(X, s, e + 1.0)
     assert not gate0.accept(Xt, st).any()
+
+
+def test_mirrored_models_land_on_mirrored_nodes():
+    """The mirror images used for training are exact only if the receivers and the
+    snapped source nodes of a model and of its mirror image are mirror images too
+    (configuration 5's mesh: Q4 200 x 200, 10 m; 64 receivers)."""
+    gen = surrogate.FEMGenerator(nelem=200, edge=10.0, pml=20, steps=10, nrec=64, mpml_fn=oracle.oracle_abc_mpml)
+    L = gen.L
+    yr = gen.y[gen.rec]
+    assert np.allclose(yr + yr[::-1], L)
+    m = surrogate.sample_models(2000, seed=3, size=L)
+    sn = gen.source_nodes(m["src_x"], m["src_depth"])
+    sm = gen.source_nodes(L - m["src_x"], m["src_depth"])
+    assert np.allclose(gen.y[sn] + gen.y[sm], L) and np.allclose(gen.x[sn], gen.x[sm])
+    X = m["X"][:5]
+    Y = np.arange(5 * 64 * 3, dtype=np.float32).reshape(5, 64 * 3)
+    Xm, Ym = surrogate.mirrored(X, Y, 64)
+    assert np.allclose(Xm[:, 11], 1.0 - X[:, 11]) and np.array_equal(np.delete(Xm, 11, 1), np.delete(X, 11, 1))
+    assert np.array_equal(Ym.reshape(5, 64, 3)[:, ::-1], Y.reshape(5, 64, 3))
