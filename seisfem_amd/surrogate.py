@@ -315,14 +315,21 @@ def mirrored(X, Y, nrec, size_col=11):
     return Xm, Ym
 
 
-def ensemble_predict(nets, X):
+def ensemble_predict(nets, X, mirror_nrec=0):
     """(mean prediction, spread, device ms): spread = rms over the members of each
-    member's rel-L2 distance from the mean."""
+    member's rel-L2 distance from the mean.  With mirror_nrec > 0 each network
+    also predicts the mirror image of every model, mirrored back (``mirrored``),
+    as one more member."""
     preds, ms = [], 0.0
+    Xm = mirrored(X, np.zeros((X.shape[0], mirror_nrec), np.float32), mirror_nrec)[0] if mirror_nrec else None
     for net in nets:
         P, t = net.predict(X)
         preds.append(P)
         ms += t
+        if mirror_nrec:
+            Pm, t = net.predict(Xm)
+            preds.append(mirrored(Xm, Pm, mirror_nrec)[1])
+            ms += t
     mean = np.mean(preds, axis=0)
     spread = np.sqrt(np.mean([rel_l2_rows(P, mean) ** 2 for P in preds], axis=0))
     return mean, spread, ms
@@ -378,15 +385,17 @@ class ErrorGate:
         return self.estimate(X, spread) <= self.threshold
 
 
-def evaluate_gated(nets, gate, Xtest, Ytest, fem_seconds_per_shot, audit_frac=0.1, min_audit=10, seed=0):
+def evaluate_gated(nets, gate, Xtest, Ytest, fem_seconds_per_shot, audit_frac=0.1, min_audit=10, seed=0,
+                   mirror_nrec=0):
     """The surrogate with the FEM as fallback and judge, on held-out models.
     Every model gets the ensemble's prediction; the gate accepts it or sends the
     model to the FEM, and the FEM also re-runs a random sample of the accepted
     models (audit_frac of them, at least min_audit) to judge the gate.  The
     held-out FEM traces are known here, so the report also gives the gate's
     true precision over all accepted models.  Combined rate = models / (device
-    time of the predictions + FEM time of the rejected and audited models)."""
-    mean, spread, ms = ensemble_predict(nets, Xtest)
+    time of the predictions + FEM time of the rejected and audited models).
+    mirror_nrec: as in ``ensemble_predict`` (the gate must be fitted the same way)."""
+    mean, spread, ms = ensemble_predict(nets, Xtest, mirror_nrec)
     err = rel_l2_rows(mean, Ytest)
     acc = gate.accept(Xtest, spread)
     n, na = Xtest.shape[0], int(acc.sum())
@@ -427,7 +436,8 @@ def main():
     ap.add_argument("--threshold", type=float, default=0.05)
     ap.add_argument("--precision", type=float, default=0.9, help="gate precision on the validation models")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--no-mirror", action="store_true", help="train without the mirrored models")
+    ap.add_argument("--no-mirror", action="store_true",
+                    help="no mirror images (neither as training models nor as extra predictions)")
     args = ap.parse_args()
 
     def log(m):
@@ -453,10 +463,12 @@ def main():
         vals.append(val)
         mss.append(ms)
     train_s = time.time() - t1
-    mean_v, spread_v, _ = ensemble_predict(nets, X[nfit:args.train])
+    # each network also predicts the mirror image of every model (mirrored back)
+    mnrec = 0 if args.no_mirror else gen.rec.size
+    mean_v, spread_v, _ = ensemble_predict(nets, X[nfit:args.train], mnrec)
     gate = ErrorGate(bar=args.threshold, precision=args.precision).fit(
         X[nfit:args.train], spread_v, rel_l2_rows(mean_v, Y[nfit:args.train]))
-    rep = evaluate_gated(nets, gate, X[args.train:], Y[args.train:], fem_s / n, seed=args.seed)
+    rep = evaluate_gated(nets, gate, X[args.train:], Y[args.train:], fem_s / n, seed=args.seed, mirror_nrec=mnrec)
     log(f"accepted {rep['accepted']:.4f} at {args.threshold} (true precision {rep['precision_true']}), "
         f"gain {rep['gain']:.4f}")
     out = {"config": f"surrogate 13-{args.hidden}x3-{Y.shape[1]}, {args.train} FEM shots Q4 {args.nelem}x"
