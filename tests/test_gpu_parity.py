@@ -811,20 +811,37 @@ def test_fp32_within_tolerance_of_fp64_oracle(prec, tol):
     assert err < tol
 
 
-@pytest.mark.parametrize("prec,sp", [(gpu.FP64, 1), (gpu.FP32_KAHAN16, 64)])
-def test_energy_guard_reports_instead_of_exiting(prec, sp):
+@pytest.mark.parametrize("prec,sp,every", [(gpu.FP64, 1, 20), (gpu.FP32_KAHAN16, 64, 20), (gpu.FP64, 1, 50),
+                                           (gpu.FP32_KAHAN16, 64, 50)])
+def test_energy_guard_reports_instead_of_exiting(prec, sp, every):
     """dt far above the stability limit: the run stops with SFG_E_ENERGY
-    (the reference calls exit(1)); kernels finish normally on inf/NaN."""
+    (the reference calls exit(1)); kernels finish normally on inf/NaN.  With
+    the default interval (50, what the bench's runs use) the blow-up is
+    reported at the first check after it, so at most 49 steps late: the
+    oracle's own loop (which checks every step) stops at step k, and the GPU
+    run names the check that covers k."""
     case = _case(host.Q4, 12, 400)
     prob = case["problem"]
-    say(f"energy guard Q4 12x12 dt x8 precision={prec} sp={sp}")
+    say(f"energy guard Q4 12x12 dt x8 precision={prec} sp={sp} every {every}")
     op = gpu.Operator(prob["node_num"], prob["csr"])
     md = gpu.Medium(op, prob["c"], prob["mass_lump"], prob["dirichlet"], prob["mpml"])
     run = gpu.Run(op, md, case["src"], 400, case["rec"], case["dt"] * 8, case["f0"], case["t0"],
-                  precision=prec, guard_every=20, shots_per_lane_group=sp)
+                  precision=prec, guard_every=every, shots_per_lane_group=sp)
     with pytest.raises(__import__("seisfem_amd").NativeError) as ei:
         run.advance(400)
     assert ei.value.code == -6
+    if prec == gpu.FP64:
+        ref = oracle.oracle_masslump(prob, case["src"][0], 402, case["rec"], case["dt"] * 8, case["f0"],
+                                     case["t0"])
+        assert ref["status"] == -6
+        # the oracle stopped in iteration it (energy[it] is its last record), its
+        # (it - 1)-th executed step; the first GPU check at or after that step
+        E = ref["energy"]
+        it = int(np.max(np.nonzero(E[:, 0] + E[:, 1])[0]))
+        k = it - 1
+        check = (k + every - 1) // every
+        msg = str(ei.value)
+        assert f"at check {check} " in msg or f"at check {check + 1} " in msg, (k, check, msg)
     run.close()
     md.close()
     op.close()
