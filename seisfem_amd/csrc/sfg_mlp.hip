@@ -427,25 +427,27 @@ int mfail(int code, const char *fmt, ...)
         if (e_ != hipSuccess) return mfail(SFG_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));     \
     } while (0)
 
-// C = A . B^T, A [M][K] and B [N][K] (or, tb, [K][N]), epilogue ep.  Tiles of
-// 128 x 128 when they give the GPU enough blocks, else 64 x 64 (four waves of
-// 64 x 64 or 32 x 32); a plain store with still too few blocks splits K (the
-// partial products meet in C by atomics).
+// C = A . B^T, A [M][K] and B [N][K] (or, tb, [K][N]), epilogue ep; a plain
+// store with too few blocks splits K (the partial products meet in C by atomics).
 hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float *C,
                 int ldc, const EpiArgs &ep, bool tb = false)
 {
     if (K % 8 || lda % 8 || ldb % 8 || M <= 0 || N <= 0 || K <= 0) return hipErrorInvalidValue;
     if (tb && N % 8) return hipErrorInvalidValue;
     if (ep.kind == EPI_MSE && !ep.H) return hipErrorInvalidValue;
-    // 128 x 128 with four waves of 64 x 64 measured faster on 256 x 32000 x 1024
-    // than eight waves of 32 x 64 or 256 x 128 tiles; for the K = 32000 product
-    // (dX of the output layer) 64 x 64 tiles split 8 ways beat 256 x 128 split 63
+    // Tiles: 256 x 128 (eight waves of 64 x 64, one block per CU) for a product whose
+    // M is one or two tiles of the batch (256 x 32000 x 1024: 28.7 us against 31.0 us
+    // for 128 x 128); 128 x 128 (four waves of 64 x 64) for other large products (dW of
+    // the output layer, M = 32000: faster than 256 x 128); 64 x 64 below 256 tiles, the
+    // K = 32000 product (dX of the output layer) split 8 ways (faster than 256 x 128
+    // split 63).  Loading three K steps ahead instead of two measured no faster.
     const bool big = (long)((M + 127) / 128) * ((N + 127) / 128) >= 256;
-    const int bm = big ? 128 : 64, bn = bm;
+    const bool wide = big && M % 256 == 0 && M <= 512;
+    const int bm = wide ? 256 : big ? 128 : 64, bn = big ? 128 : 64;
     const long tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
     int splits = 1;
     const int ksteps = (K + BK - 1) / BK;
-    if (ep.kind == EPI_STORE && tiles < 256) splits = (int)std::min<long>(ksteps, (512 + tiles - 1) / tiles);
+    if (ep.kind == EPI_STORE && tiles < 256 && !wide) splits = (int)std::min<long>(ksteps, (512 + tiles - 1) / tiles);
     const int kchunk = ((ksteps + splits - 1) / splits) * BK;
     splits = (K + kchunk - 1) / kchunk;
     if (splits > 1) {
@@ -456,7 +458,14 @@ hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int lda,
 #define SFG_GEMM(BMV, TBV)                                                                                        \
     hipLaunchKernelGGL((gemm_nt_kernel<BMV, BMV, 4, 2, TBV>), g, dim3(256), 0, st, M, N, K, kchunk, A, lda, B, ldb, C, \
                        ldc, ep)
-    if (big) {
+    if (wide) {
+        if (tb)
+            hipLaunchKernelGGL((gemm_nt_kernel<256, 128, 8, 2, true>), g, dim3(512), 0, st, M, N, K, kchunk, A, lda, B,
+                               ldb, C, ldc, ep);
+        else
+            hipLaunchKernelGGL((gemm_nt_kernel<256, 128, 8, 2, false>), g, dim3(512), 0, st, M, N, K, kchunk, A, lda, B,
+                               ldb, C, ldc, ep);
+    } else if (big) {
         if (tb) SFG_GEMM(128, true); else SFG_GEMM(128, false);
     } else {
         if (tb) SFG_GEMM(64, true); else SFG_GEMM(64, false);
