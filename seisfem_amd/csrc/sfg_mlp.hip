@@ -137,25 +137,44 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
     // .., and writes each column's LB values as one piece of the tile row.  In
     // that tile the 16-byte chunk j of row n sits at chunk j ^ (n / 8 % 8), so
     // that the 8 columns of a thread and their neighbours' spread over the banks.
-    auto load_rows = [&](const uint16_t *X, int ld, int lim, int r0, int q, int k0) __attribute__((always_inline)) {
+    // Every load is issued, from a clamped in-bounds address when its piece lies
+    // past an edge, and such a piece is replaced by zeros when it is stored to
+    // LDS (bit i of ok[set] marks piece i valid): a load skipped by a branch, or
+    // a select right after it, would make the wait before a register set's store
+    // wait for every load in flight, the next steps' included.
+    uint32_t ok[2] = {0u, 0u};
+    auto load_rows = [&](const uint16_t *X, int ld, int lim, int r0, int q, int k0, bool &valid)
+        __attribute__((always_inline)) {
         const int row = q >> 3, kk = k0 + (q & 7) * 8, g = r0 + row;
-        return (g < lim && kk < ke) ? *reinterpret_cast<const uint4 *>(X + (size_t)g * ld + kk) : make_uint4(0, 0, 0, 0);
+        valid = g < lim && kk < ke;
+        return *reinterpret_cast<const uint4 *>(X + (size_t)min(g, lim - 1) * ld + min(kk, ke - 8));
     };
     const int tcg = tid % (BN / 8), tkg = tid / (BN / 8);
     auto fetch = [&](auto S, int k0) __attribute__((always_inline)) {
         constexpr int st = decltype(S)::value;
+        uint32_t m = 0;
+        bool valid;
 #pragma unroll
-        for (int i = 0; i < LA; ++i) ra[st][i] = load_rows(A, lda, M, m0, tid + NT * i, k0);
+        for (int i = 0; i < LA; ++i) {
+            ra[st][i] = load_rows(A, lda, M, m0, tid + NT * i, k0, valid);
+            m |= (uint32_t)valid << i;
+        }
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
             if constexpr (TB) {
                 const int kk = k0 + tkg * LB + i, g = n0 + tcg * 8;
-                rb[st][i] = (g < N && kk < ke) ? *reinterpret_cast<const uint4 *>(B + (size_t)kk * ldb + g)
-                                               : make_uint4(0, 0, 0, 0);
+                rb[st][i] = *reinterpret_cast<const uint4 *>(B + (size_t)min(kk, ke - 1) * ldb + min(g, N - 8));
+                valid = g < N && kk < ke;
             } else {
-                rb[st][i] = load_rows(B, ldb, N, n0, tid + NT * i, k0);
+                rb[st][i] = load_rows(B, ldb, N, n0, tid + NT * i, k0, valid);
             }
+            m |= (uint32_t)valid << (LA + i);
         }
+        ok[st] = m;
+    };
+    auto kept = [&](const uint4 &v, uint32_t m, int bit) __attribute__((always_inline)) {
+        const uint32_t z = 0u - ((m >> bit) & 1u);   // all ones for a valid piece (no branch)
+        return make_uint4(v.x & z, v.y & z, v.z & z, v.w & z);
     };
     auto half = [](const uint4 &v, int e) __attribute__((always_inline)) {
         const uint32_t w = e < 2 ? v.x : e < 4 ? v.y : e < 6 ? v.z : v.w;
@@ -166,43 +185,52 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
             const int q = tid + NT * i;
-            *reinterpret_cast<uint4 *>(&As[buf][(q >> 3) * LDK + (q & 7) * 8]) = ra[st][i];
+            *reinterpret_cast<uint4 *>(&As[buf][(q >> 3) * LDK + (q & 7) * 8]) = kept(ra[st][i], ok[st], i);
         }
         if constexpr (TB) {
             static_assert(LB == 1 || LB == 2 || LB == 4, "k-major piece");
+            uint4 rv[LB];
+#pragma unroll
+            for (int i = 0; i < LB; ++i) rv[i] = kept(rb[st][i], ok[st], LA + i);
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int kq = tkg * LB;
                 uint16_t *dst = &Bs[buf][(tcg * 8 + e) * LDK + (((kq >> 3) ^ (tcg & 7)) << 3) + (kq & 7)];
                 if constexpr (LB == 1) {
-                    *dst = (uint16_t)half(rb[st][0], e);
+                    *dst = (uint16_t)half(rv[0], e);
                 } else if constexpr (LB == 2) {
-                    *reinterpret_cast<uint32_t *>(dst) = half(rb[st][0], e) | (half(rb[st][1], e) << 16);
+                    *reinterpret_cast<uint32_t *>(dst) = half(rv[0], e) | (half(rv[1], e) << 16);
                 } else {
-                    *reinterpret_cast<uint2 *>(dst) = make_uint2(half(rb[st][0], e) | (half(rb[st][1], e) << 16),
-                                                                 half(rb[st][2], e) | (half(rb[st][3], e) << 16));
+                    *reinterpret_cast<uint2 *>(dst) = make_uint2(half(rv[0], e) | (half(rv[1], e) << 16),
+                                                                 half(rv[2], e) | (half(rv[3], e) << 16));
                 }
             }
         } else {
 #pragma unroll
             for (int i = 0; i < LB; ++i) {
                 const int q = tid + NT * i;
-                *reinterpret_cast<uint4 *>(&Bs[buf][(q >> 3) * LDK + (q & 7) * 8]) = rb[st][i];
+                *reinterpret_cast<uint4 *>(&Bs[buf][(q >> 3) * LDK + (q & 7) * 8]) = kept(rb[st][i], ok[st], LA + i);
             }
         }
     };
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
-    const int nk = (ke - kb + BK - 1) / BK;
-    if (nk > 0) {
-        fetch(I0{}, kb);
-        if (nk > 1) fetch(I1{}, kb + BK);
-        stash(I0{}, 0);
-    }
+    // The loop has no branch but its own: the step count is rounded up to even
+    // (an extra step multiplies tiles of zeros, which adds exact zeros), and the
+    // fetches past the last step load clamped addresses whose pieces are zeroed.
+    // The compiler can then wait before a store for the older register set's
+    // loads only, with the newer set's still in flight.
+    const int nk = ((ke - kb + BK - 1) / BK + 1) & ~1;
+    fetch(I0{}, kb);
+    fetch(I1{}, kb + BK);
+    stash(I0{}, 0);
     __syncthreads();
     auto step = [&](int t, auto S) __attribute__((always_inline)) {
         constexpr int buf = decltype(S)::value;
-        if (t + 2 < nk) fetch(S, kb + (t + 2) * BK);
+        fetch(S, kb + (t + 2) * BK);
+        // keep the loads at the top of the step (the scheduler would sink them
+        // below the products and the other set's store in the first step of a pair)
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             // operands of v_mfma_f32_32x32x16_bf16: lane l holds row (col) l % 32 and
@@ -224,12 +252,12 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
                 for (int b = 0; b < TN; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
         }
-        if (t + 1 < nk) stash(std::integral_constant<int, 1 - buf>{}, 1 - buf);
+        stash(std::integral_constant<int, 1 - buf>{}, 1 - buf);
         __syncthreads();
     };
     for (int t = 0; t < nk; t += 2) {
         step(t, I0{});
-        if (t + 1 < nk) step(t + 1, I1{});
+        step(t + 1, I1{});
     }
     // epilogue: lane l, register r -> row (r/4)*8 + (l/32)*4 + r%4, column l%32.
     // Per 32 x 32 tile, every load an epilogue needs is issued before any of its
@@ -251,6 +279,33 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
             for (int r = 0; r < 16; ++r)
                 if (col < N && SFG_ROW(r) < M) atomicAdd(&C[(size_t)SFG_ROW(r) * ldc + col], t[r]);
         });
+    } else if (ep.kind == EPI_STORE && N % 4 == 0 && ldc % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) {
+        // Each wave turns its tile rows round in LDS (the A buffers, idle after the
+        // last step's barrier), 32 rows at a time, so that C is written as whole
+        // rows of 16-byte pieces rather than as 4-byte elements: a wave's 32 x WC
+        // slab is written to LDS element by element in the register layout, then
+        // read back and stored along the rows.
+        constexpr int WC = BN / WN, SLD = WC + (WC >= 64 ? 8 : 4); // floats per staged row
+        constexpr int PR = WC / 4, PL = 32 * PR / 64;               // pieces per row, per lane
+        static_assert(NW * 32 * SLD * 4 <= (int)sizeof(As), "staging slab fits in the A buffers");
+        float *S = reinterpret_cast<float *>(&As[0][0]) + wave * 32 * SLD;
+        const int rw = m0 + wm * (BM / WM), cw = n0 + wn * WC;
+#pragma unroll
+        for (int a = 0; a < TM; ++a) {
+#pragma unroll
+            for (int b = 0; b < TN; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    S[((r >> 2) * 8 + (lane >> 5) * 4 + (r & 3)) * SLD + b * 32 + (lane & 31)] = acc[a][b][r];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < PL; ++i) {
+                const int q = lane + 64 * i, row = rw + a * 32 + q / PR, col = cw + (q % PR) * 4;
+                const float4 v = *reinterpret_cast<const float4 *>(&S[(q / PR) * SLD + (q % PR) * 4]);
+                if (row < M && col < N) *reinterpret_cast<float4 *>(&C[(size_t)row * ldc + col]) = v;
+            }
+            __syncthreads();
+        }
     } else if (ep.kind == EPI_STORE) {
         tiles([&](const f32x16 &t, int col, int rb) __attribute__((always_inline)) {
 #pragma unroll
