@@ -279,17 +279,30 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
             for (int r = 0; r < 16; ++r)
                 if (col < N && SFG_ROW(r) < M) atomicAdd(&C[(size_t)SFG_ROW(r) * ldc + col], t[r]);
         });
-    } else if (ep.kind == EPI_STORE && N % 4 == 0 && ldc % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0) {
+    } else if (N % 4 == 0 && ldc % 4 == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0 &&
+               (ep.kind == EPI_STORE || (reinterpret_cast<uintptr_t>(ep.bias) & 15) == 0) &&
+               (!ep.H || (ep.ldh % 4 == 0 && (reinterpret_cast<uintptr_t>(ep.H) & 7) == 0)) &&
+               (ep.kind != EPI_MSE ||
+                (ep.ldt % 4 == 0 && ep.nvalid % 4 == 0 && (reinterpret_cast<uintptr_t>(ep.T) & 15) == 0))) {
         // Each wave turns its tile rows round in LDS (the A buffers, idle after the
-        // last step's barrier), 32 rows at a time, so that C is written as whole
-        // rows of 16-byte pieces rather than as 4-byte elements: a wave's 32 x WC
-        // slab is written to LDS element by element in the register layout, then
-        // read back and stored along the rows.
+        // last step's barrier), 32 rows at a time, so that the epilogue reads and
+        // writes whole rows in 4-element pieces (16 bytes of C, bias and targets,
+        // 8 bytes of H) rather than single elements: a wave's 32 x WC slab is
+        // written to LDS element by element in the register layout, then read back
+        // along the rows.  The arithmetic per element is that of the paths below.
         constexpr int WC = BN / WN, SLD = WC + (WC >= 64 ? 8 : 4); // floats per staged row
-        constexpr int PR = WC / 4, PL = 32 * PR / 64;               // pieces per row, per lane
+        constexpr int PR = WC / 4, PL = 32 * PR / 64, RS = 64 / PR;   // pieces per row and per lane, rows apart
         static_assert(NW * 32 * SLD * 4 <= (int)sizeof(As), "staging slab fits in the A buffers");
+        static_assert(64 % PR == 0, "a lane keeps its columns");
         float *S = reinterpret_cast<float *>(&As[0][0]) + wave * 32 * SLD;
-        const int rw = m0 + wm * (BM / WM), cw = n0 + wn * WC;
+        // lane: columns col .. col + 3 of rows r0 + a * 32 + i * RS
+        const int pc = lane % PR, pr = lane / PR;
+        const int col = n0 + wn * WC + pc * 4, r0 = m0 + wm * (BM / WM) + pr;
+        const bool mse = ep.kind == EPI_MSE, act = ep.kind == EPI_BIAS_GELU;
+        const int nv = mse ? ep.nvalid : N; // a multiple of 4: a piece is all valid or all not
+        float4 bc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ep.kind != EPI_STORE) bc = *reinterpret_cast<const float4 *>(&ep.bias[min(col, N - 4)]);
+        double lsum = 0.0;
 #pragma unroll
         for (int a = 0; a < TM; ++a) {
 #pragma unroll
@@ -297,14 +310,59 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
 #pragma unroll
                 for (int r = 0; r < 16; ++r)
                     S[((r >> 2) * 8 + (lane >> 5) * 4 + (r & 3)) * SLD + b * 32 + (lane & 31)] = acc[a][b][r];
+            // targets: the pass's sample indices, then its 16-byte target pieces, all
+            // from in-bounds addresses (clamped; what lies past an edge is not used),
+            // all in flight together
+            float4 tv[PL];
+            if (mse) {
+                int ti[PL];
+#pragma unroll
+                for (int i = 0; i < PL; ++i) ti[i] = ep.tidx[min(r0 + a * 32 + i * RS, M - 1)];
+#pragma unroll
+                for (int i = 0; i < PL; ++i)
+                    tv[i] = *reinterpret_cast<const float4 *>(ep.T + (size_t)ti[i] * ep.ldt + min(col, nv - 4));
+            }
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < PL; ++i) {
-                const int q = lane + 64 * i, row = rw + a * 32 + q / PR, col = cw + (q % PR) * 4;
-                const float4 v = *reinterpret_cast<const float4 *>(&S[(q / PR) * SLD + (q % PR) * 4]);
-                if (row < M && col < N) *reinterpret_cast<float4 *>(&C[(size_t)row * ldc + col]) = v;
+                const int row = r0 + a * 32 + i * RS;
+                const float4 v = *reinterpret_cast<const float4 *>(&S[(pr + i * RS) * SLD + pc * 4]);
+                if (row >= M || col >= N) continue;
+                float4 o = v;
+                if (ep.kind != EPI_STORE) o = make_float4(v.x + bc.x, v.y + bc.y, v.z + bc.z, v.w + bc.w);
+                if (mse) {
+                    const float d[4] = {o.x - tv[i].x, o.y - tv[i].y, o.z - tv[i].z, o.w - tv[i].w};
+                    float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                    if (col < nv) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            lsum += (double)d[e] * d[e];
+                            g[e] = ep.scale * d[e];
+                        }
+                    }
+                    o = make_float4(g[0], g[1], g[2], g[3]);
+                }
+                *reinterpret_cast<float4 *>(&C[(size_t)row * ldc + col]) = o;
+                if (ep.H) {
+                    const float h[4] = {act ? gelu(o.x) : o.x, act ? gelu(o.y) : o.y, act ? gelu(o.z) : o.z,
+                                        act ? gelu(o.w) : o.w};
+                    *reinterpret_cast<uint2 *>(&ep.H[(size_t)row * ep.ldh + col]) =
+                        make_uint2(f2bf(h[0]) | ((uint32_t)f2bf(h[1]) << 16), f2bf(h[2]) | ((uint32_t)f2bf(h[3]) << 16));
+                }
             }
             __syncthreads();
+        }
+        if (mse) {
+            // one atomic per block: the waves' sums meet in LDS (free after the last pass's barrier)
+            for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
+            double *ws = reinterpret_cast<double *>(&Bs[0][0]);
+            if (lane == 0) ws[wave] = lsum;
+            __syncthreads();
+            if (tid == 0) {
+                double t = 0.0;
+                for (int w = 0; w < NW; ++w) t += ws[w];
+                atomicAdd(ep.loss, t);
+            }
         }
     } else if (ep.kind == EPI_STORE) {
         tiles([&](const f32x16 &t, int col, int rb) __attribute__((always_inline)) {

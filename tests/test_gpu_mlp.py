@@ -36,16 +36,19 @@ def _net(batch=32, sizes=(13, 64, 64, 64, 96), seed=3):
     return surrogate.MLP(sizes[0], sizes[1], len(sizes) - 2, sizes[-1], batch=batch, seed=seed)
 
 
-def test_gradients_match_oracle():
-    net = _net()
+# batch 256 takes the 256-row output tiles; 97 outputs (padded to 128) the
+# element-wise loss epilogue, 96 the row-piece one
+@pytest.mark.parametrize("batch,nout", [(32, 96), (256, 96), (32, 97)])
+def test_gradients_match_oracle(batch, nout):
+    net = _net(batch=batch, sizes=(13, 64, 64, 64, nout))
     rng = np.random.default_rng(0)
-    X = rng.uniform(0, 1, (32, 13)).astype(np.float32)
-    T = rng.standard_normal((32, 96)).astype(np.float32)
+    X = rng.uniform(0, 1, (batch, 13)).astype(np.float32)
+    T = rng.standard_normal((batch, nout)).astype(np.float32)
     Ws, bs = zip(*[net.layer(l) for l in range(net.nlayers)])
     Ws = [w.astype(np.float64) for w in Ws]
     bs = [b.astype(np.float64) for b in bs]
     loss_ref, gW, gb = mo.loss_and_grads(Ws, bs, X, T, round_bf16=True)
-    say("mlp gradients 13-64-64-64-96 batch 32")
+    say(f"mlp gradients 13-64-64-64-{nout} batch {batch}")
     for l in range(net.nlayers):
         loss, g, b = net.gradients(X, T, l)
         assert abs(loss - loss_ref) <= 1e-4 * loss_ref
