@@ -498,26 +498,50 @@ __global__ __launch_bounds__(256) void bias_grad_kernel(int batch, int cols, int
 }
 
 // Adam on FP32 master weights [rows][ld] (valid columns < cols, ld a multiple
-// of 4), four weights per thread and trip; refreshes the bf16 copy w16
-__global__ void adam_kernel(int rows, int cols, int ld, float *w, const float *g, float *m, float *v, AdamArgs a,
-                            uint16_t *w16)
+// of 4), four weights per piece, two pieces per thread and trip; refreshes the
+// bf16 copy w16.  Every byte is touched once: non-temporal loads and stores,
+// and both pieces' loads (from clamped, in-bounds indices) are issued before
+// any store, so that a thread keeps two pieces' reads in flight.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ void adam_kernel(int rows, int cols, int ld, float *__restrict__ w, const float *__restrict__ g,
+                            float *__restrict__ m, float *__restrict__ v, AdamArgs a, uint16_t *__restrict__ w16)
 {
-    const size_t total4 = (size_t)rows * ld / 4;
-    for (size_t q4 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q4 < total4; q4 += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)((q4 * 4) % ld);
-        float4 W = reinterpret_cast<float4 *>(w)[q4], Mm = reinterpret_cast<float4 *>(m)[q4];
-        float4 V = reinterpret_cast<float4 *>(v)[q4];
-        const float4 G = reinterpret_cast<const float4 *>(g)[q4];
-        float *pw = &W.x, *pm = &Mm.x, *pv = &V.x;
-        const float *pg = &G.x;
+    constexpr int U = 2;
+    const size_t total4 = (size_t)rows * ld / 4, stride = (size_t)gridDim.x * blockDim.x;
+    f32x4 *w4 = reinterpret_cast<f32x4 *>(w), *m4 = reinterpret_cast<f32x4 *>(m), *v4 = reinterpret_cast<f32x4 *>(v);
+    const f32x4 *g4 = reinterpret_cast<const f32x4 *>(g);
+    uint2 *h4 = reinterpret_cast<uint2 *>(w16);
+    for (size_t q0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q0 < total4; q0 += U * stride) {
+        f32x4 W[U], Mm[U], V[U], G[U];
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-            if (c + e < cols) pw[e] = adam_update(pw[e], pg[e], pm[e], pv[e], a);
-        reinterpret_cast<float4 *>(w)[q4] = W;
-        reinterpret_cast<float4 *>(m)[q4] = Mm;
-        reinterpret_cast<float4 *>(v)[q4] = V;
-        reinterpret_cast<uint2 *>(w16)[q4] = make_uint2((uint32_t)f2bf(W.x) | ((uint32_t)f2bf(W.y) << 16),
-                                                        (uint32_t)f2bf(W.z) | ((uint32_t)f2bf(W.w) << 16));
+        for (int u = 0; u < U; ++u) {
+            const size_t q = min(q0 + u * stride, total4 - 1);
+            W[u] = __builtin_nontemporal_load(&w4[q]);
+            Mm[u] = __builtin_nontemporal_load(&m4[q]);
+            V[u] = __builtin_nontemporal_load(&v4[q]);
+            G[u] = __builtin_nontemporal_load(&g4[q]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t q = q0 + u * stride;
+            if (q >= total4) break;
+            const int c = (int)((q * 4) % ld);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (c + e < cols) {
+                    float pm = Mm[u][e], pv = V[u][e];
+                    W[u][e] = adam_update(W[u][e], G[u][e], pm, pv, a);
+                    Mm[u][e] = pm;
+                    V[u][e] = pv;
+                }
+            __builtin_nontemporal_store(W[u], &w4[q]);
+            __builtin_nontemporal_store(Mm[u], &m4[q]);
+            __builtin_nontemporal_store(V[u], &v4[q]);
+            const uint2 h = make_uint2((uint32_t)f2bf(W[u][0]) | ((uint32_t)f2bf(W[u][1]) << 16),
+                                       (uint32_t)f2bf(W[u][2]) | ((uint32_t)f2bf(W[u][3]) << 16));
+            __builtin_nontemporal_store(h.x, &h4[q].x);
+            __builtin_nontemporal_store(h.y, &h4[q].y);
+        }
     }
 }
 
