@@ -417,16 +417,39 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
 // C[r][c] = 0 for r < rows, c < cols (row stride ld)
 __global__ void fill_zero_f32_kernel(int rows, int cols, int ld, float *p)
 {
-    const size_t n = (size_t)rows * cols;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        p[(i / cols) * ld + i % cols] = 0.0f;
+    if (ld == cols && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+        const size_t step = (size_t)gridDim.x * gridDim.y * blockDim.x;
+        const size_t t0 = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+        // contiguous rows: float4 stores, then the last n % 4 elements
+        const size_t n = (size_t)rows * cols, n4 = n / 4;
+        for (size_t i = t0; i < n4; i += step) reinterpret_cast<float4 *>(p)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (t0 < n - 4 * n4) p[4 * n4 + t0] = 0.0f;
+        return;
+    }
+    for (int r = blockIdx.y; r < rows; r += gridDim.y)
+        for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += gridDim.x * blockDim.x) p[(size_t)r * ld + c] = 0.0f;
 }
 
-// dst[c][r] = src[r][c] for r < rows, c < cols (bf16)
-__global__ void transpose_bf16_kernel(int rows, int cols, const uint16_t *src, int lds, uint16_t *dst, int ldd)
+// dst[c][r] = src[r][c] for r < rows, c < cols (bf16), for two such matrices in
+// one launch (both operands of a weight gradient): blocks 0 .. nb0 - 1 take the
+// 32 x 32 tiles of the first, row of tiles by row, the rest those of the second
+struct Transpose {
+    int rows, cols;
+    const uint16_t *src;
+    int lds;
+    uint16_t *dst;
+    int ldd;
+};
+__global__ void transpose_bf16_kernel(Transpose t0, Transpose t1, int nb0)
 {
+    const bool second = (int)blockIdx.x >= nb0;
+    const Transpose &t = second ? t1 : t0;
+    const int rows = t.rows, cols = t.cols, lds = t.lds, ldd = t.ldd;
+    const uint16_t *src = t.src;
+    uint16_t *dst = t.dst;
+    const int b = (int)blockIdx.x - (second ? nb0 : 0), tx = (cols + 31) / 32;
+    const int c0 = (b % tx) * 32, r0 = (b / tx) * 32;
     __shared__ uint16_t tile[32][33];
-    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
     for (int j = threadIdx.y; j < 32; j += blockDim.y) {
         const int r = r0 + j, c = c0 + threadIdx.x;
         tile[j][threadIdx.x] = (r < rows && c < cols) ? src[(size_t)r * lds + c] : (uint16_t)0;
@@ -611,10 +634,10 @@ hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int lda,
     return hipGetLastError();
 }
 
-hipError_t transpose(hipStream_t st, int rows, int cols, const uint16_t *src, int lds, uint16_t *dst, int ldd)
+hipError_t transpose2(hipStream_t st, const Transpose &a, const Transpose &b)
 {
-    const dim3 g((cols + 31) / 32, (rows + 31) / 32);
-    hipLaunchKernelGGL(transpose_bf16_kernel, g, dim3(32, 8), 0, st, rows, cols, src, lds, dst, ldd);
+    const int nb0 = ((a.cols + 31) / 32) * ((a.rows + 31) / 32), nb1 = ((b.cols + 31) / 32) * ((b.rows + 31) / 32);
+    hipLaunchKernelGGL(transpose_bf16_kernel, dim3(nb0 + nb1), dim3(32, 8), 0, st, a, b, nb0);
     return hipGetLastError();
 }
 
@@ -819,8 +842,8 @@ static int backward(sfg_mlp *m, const AdamArgs *ad = nullptr)
         // dW = dz^T . h from transposed copies (A = dz^T [pout][B], B = h^T [pin][B]); K = batch.
         // Reading both operands k-major instead measured 2.4x slower (transposing into LDS
         // element by element costs as much as the products at this small K)
-        MCHK(transpose(m->st, B, m->pout[l], m->dz16[l], m->pout[l], m->dzT16[l], B));
-        MCHK(transpose(m->st, B, m->pin[l], m->h16[l], m->pin[l], m->hT16[l], B));
+        MCHK(transpose2(m->st, Transpose{B, m->pout[l], m->dz16[l], m->pout[l], m->dzT16[l], B},
+                        Transpose{B, m->pin[l], m->h16[l], m->pin[l], m->hT16[l], B}));
         MCHK(gemm(m->st, m->pout[l], m->pin[l], B, m->dzT16[l], B, m->hT16[l], B, m->G[l], m->pin[l], EpiArgs{}));
         if (ad) {
             const size_t n4 = (size_t)m->dout[l] * m->pin[l] / 4;
