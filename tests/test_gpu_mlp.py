@@ -21,7 +21,11 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("M,N,K", [(70, 200, 45), (256, 1024, 1024), (128, 32000 // 8, 256), (33, 17, 13)])
+# 256 x 32768 and 512 x 8192 take the 256-row tiles (one and two along M; >= 256
+# tiles of 128 x 128), 256 x 1024 and 128 x 4000 64 x 64 tiles with K split over
+# blocks, 70 x 200 partial 64 x 64 tiles, 33 x 17 the element-wise epilogue
+@pytest.mark.parametrize("M,N,K", [(70, 200, 45), (256, 1024, 1024), (256, 32768, 128), (512, 8192, 256),
+                                   (128, 32000 // 8, 256), (33, 17, 13)])
 def test_gemm_bf16_matches_float64_of_bf16_operands(M, N, K):
     rng = np.random.default_rng(M + N + K)
     A = rng.standard_normal((M, K)).astype(np.float32)
@@ -36,9 +40,9 @@ def _net(batch=32, sizes=(13, 64, 64, 64, 96), seed=3):
     return surrogate.MLP(sizes[0], sizes[1], len(sizes) - 2, sizes[-1], batch=batch, seed=seed)
 
 
-# batch 256 takes the 256-row output tiles; 97 outputs (padded to 128) the
-# element-wise loss epilogue, 96 the row-piece one
-@pytest.mark.parametrize("batch,nout", [(32, 96), (256, 96), (32, 97)])
+# batch 256 with 32000 outputs takes the 256-row output tiles; 97 and 32001 outputs
+# (padded to 128 and 32032) the element-wise loss epilogue, 96 and 32000 the row-piece one
+@pytest.mark.parametrize("batch,nout", [(32, 96), (32, 97), (256, 32000), (256, 32001)])
 def test_gradients_match_oracle(batch, nout):
     net = _net(batch=batch, sizes=(13, 64, 64, 64, nout))
     rng = np.random.default_rng(0)
