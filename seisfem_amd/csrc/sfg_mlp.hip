@@ -430,6 +430,23 @@ __global__ void fill_zero_f32_kernel(int rows, int cols, int ld, float *p)
         for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += gridDim.x * blockDim.x) p[(size_t)r * ld + c] = 0.0f;
 }
 
+// Zeroes up to 8 contiguous, 16-byte aligned float buffers in one launch
+// (blockIdx.y picks the buffer).
+struct ZeroList {
+    float *p[8];
+    size_t n[8];
+    int count;
+};
+__global__ void zero_list_kernel(ZeroList z)
+{
+    if ((int)blockIdx.y >= z.count) return;
+    float *p = z.p[blockIdx.y];
+    const size_t n = z.n[blockIdx.y], n4 = n / 4, step = (size_t)gridDim.x * blockDim.x;
+    const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (size_t i = t0; i < n4; i += step) reinterpret_cast<float4 *>(p)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (t0 < n - 4 * n4) p[4 * n4 + t0] = 0.0f;
+}
+
 // dst[c][r] = src[r][c] for r < rows, c < cols (bf16), for two such matrices in
 // one launch (both operands of a weight gradient): blocks 0 .. nb0 - 1 take the
 // 32 x 32 tiles of the first, row of tiles by row, the rest those of the second
@@ -589,31 +606,46 @@ int mfail(int code, const char *fmt, ...)
 
 // C = A . B^T, A [M][K] and B [N][K] (or, tb, [K][N]), epilogue ep; a plain
 // store with too few blocks splits K (the partial products meet in C by atomics).
+// Tiles: 256 x 128 (eight waves of 64 x 64, one block per CU) for a product whose M
+// is one or two tiles of the batch (256 x 32000 x 1024: 28.7 us against 31.0 us for
+// 128 x 128); 128 x 128 (four waves of 64 x 64) for other large products (dW of the
+// output layer, M = 32000: faster than 256 x 128); 64 x 64 below 256 tiles, the
+// K = 32000 product (dX of the output layer) split 8 ways (faster than 256 x 128 split
+// 63, and than 128 x 128 split 32: 0.520 against 0.511 ms per training step).
+// Loading three K steps ahead instead of two measured no faster.
+struct GemmShape {
+    bool big, wide;
+    int bm, bn, splits, kchunk;
+};
+GemmShape gemm_shape(int M, int N, int K, int kind)
+{
+    GemmShape g;
+    g.big = (long)((M + 127) / 128) * ((N + 127) / 128) >= 256;
+    g.wide = g.big && M % 256 == 0 && M <= 512;
+    g.bm = g.wide ? 256 : g.big ? 128 : 64;
+    g.bn = g.big ? 128 : 64;
+    const long tiles = (long)((M + g.bm - 1) / g.bm) * ((N + g.bn - 1) / g.bn);
+    const int ksteps = (K + BK - 1) / BK;
+    int splits = 1;
+    if (kind == EPI_STORE && tiles < 256 && !g.wide) splits = (int)std::min<long>(ksteps, (512 + tiles - 1) / tiles);
+    g.kchunk = ((ksteps + splits - 1) / splits) * BK;
+    g.splits = (K + g.kchunk - 1) / g.kchunk;
+    return g;
+}
+
+// C = A . B^T with the epilogue ep.  A product split over K adds its partial
+// products into C: zeroed here first, unless the caller has (zeroed).
 hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float *C,
-                int ldc, const EpiArgs &ep, bool tb = false)
+                int ldc, const EpiArgs &ep, bool tb = false, bool zeroed = false)
 {
     if (K % 8 || lda % 8 || ldb % 8 || M <= 0 || N <= 0 || K <= 0) return hipErrorInvalidValue;
     if (tb && N % 8) return hipErrorInvalidValue;
     if (ep.kind == EPI_MSE && !ep.H) return hipErrorInvalidValue;
-    // Tiles: 256 x 128 (eight waves of 64 x 64, one block per CU) for a product whose
-    // M is one or two tiles of the batch (256 x 32000 x 1024: 28.7 us against 31.0 us
-    // for 128 x 128); 128 x 128 (four waves of 64 x 64) for other large products (dW of
-    // the output layer, M = 32000: faster than 256 x 128); 64 x 64 below 256 tiles, the
-    // K = 32000 product (dX of the output layer) split 8 ways (faster than 256 x 128
-    // split 63, and than 128 x 128 split 32: 0.520 against 0.511 ms per training step).  Loading three K steps ahead instead of two measured no faster.
-    const bool big = (long)((M + 127) / 128) * ((N + 127) / 128) >= 256;
-    const bool wide = big && M % 256 == 0 && M <= 512;
-    const int bm = wide ? 256 : big ? 128 : 64, bn = big ? 128 : 64;
-    const long tiles = (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
-    int splits = 1;
-    const int ksteps = (K + BK - 1) / BK;
-    if (ep.kind == EPI_STORE && tiles < 256 && !wide) splits = (int)std::min<long>(ksteps, (512 + tiles - 1) / tiles);
-    const int kchunk = ((ksteps + splits - 1) / splits) * BK;
-    splits = (K + kchunk - 1) / kchunk;
-    if (splits > 1) {
-        // the partial products are added into C
+    const GemmShape gs = gemm_shape(M, N, K, ep.kind);
+    const bool wide = gs.wide, big = gs.big;
+    const int bm = gs.bm, bn = gs.bn, splits = gs.splits, kchunk = gs.kchunk;
+    if (splits > 1 && !zeroed)
         hipLaunchKernelGGL(fill_zero_f32_kernel, dim3(1024), dim3(256), 0, st, M, N, ldc, C);
-    }
     const dim3 g((N + bn - 1) / bn, (M + bm - 1) / bm, splits);
 #define SFG_GEMM(BMV, TBV)                                                                                        \
     hipLaunchKernelGGL((gemm_nt_kernel<BMV, BMV, 4, 2, TBV>), g, dim3(256), 0, st, M, N, K, kchunk, A, lda, B, ldb, C, \
@@ -825,11 +857,31 @@ static int forward(sfg_mlp *m, const float *Y = nullptr, const int *idx = nullpt
 static int backward(sfg_mlp *m, const AdamArgs *ad = nullptr)
 {
     const int B = m->batch;
+    // the products split over K (dX of most layers, dW of narrow ones) accumulate
+    // into their outputs: all of those are zeroed by one launch up front
+    ZeroList zl{};
+    std::vector<char> zx(m->nl, 0), zw(m->nl, 0);
+    for (int l = m->nl - 1; l >= 0 && zl.count < 7; --l) {
+        if (l > 0 && gemm_shape(B, m->pin[l], m->pout[l], EPI_STORE).splits > 1) {
+            zl.p[zl.count] = m->dh[l];
+            zl.n[zl.count++] = (size_t)B * m->pin[l];
+            zx[l] = 1;
+        }
+        if (gemm_shape(m->pout[l], m->pin[l], B, EPI_STORE).splits > 1) {
+            zl.p[zl.count] = m->G[l];
+            zl.n[zl.count++] = (size_t)m->pout[l] * m->pin[l];
+            zw[l] = 1;
+        }
+    }
+    if (zl.count) {
+        hipLaunchKernelGGL(zero_list_kernel, dim3(256, zl.count), dim3(256), 0, m->st, zl);
+        MCHK(hipGetLastError());
+    }
     for (int l = m->nl - 1; l >= 0; --l) {
         if (l > 0) {
             // dh = dz . W (W [pout][pin] read k-major as stored) ; dz_{l-1} = dh * gelu'(z_{l-1})
             MCHK(gemm(m->st, B, m->pin[l], m->pout[l], m->dz16[l], m->pout[l], m->W16[l], m->pin[l], m->dh[l],
-                      m->pin[l], EpiArgs{}, true));
+                      m->pin[l], EpiArgs{}, true, zx[l]));
             const size_t n = (size_t)B * m->pin[l];
             hipLaunchKernelGGL(gelu_back_kernel, dim3(1024), dim3(256), 0, m->st, n, m->dh[l], m->z[l - 1],
                                m->dz[l - 1], m->dz16[l - 1]);
@@ -844,7 +896,8 @@ static int backward(sfg_mlp *m, const AdamArgs *ad = nullptr)
         // element by element costs as much as the products at this small K)
         MCHK(transpose2(m->st, Transpose{B, m->pout[l], m->dz16[l], m->pout[l], m->dzT16[l], B},
                         Transpose{B, m->pin[l], m->h16[l], m->pin[l], m->hT16[l], B}));
-        MCHK(gemm(m->st, m->pout[l], m->pin[l], B, m->dzT16[l], B, m->hT16[l], B, m->G[l], m->pin[l], EpiArgs{}));
+        MCHK(gemm(m->st, m->pout[l], m->pin[l], B, m->dzT16[l], B, m->hT16[l], B, m->G[l], m->pin[l], EpiArgs{}, false,
+                  zw[l]));
         if (ad) {
             const size_t n4 = (size_t)m->dout[l] * m->pin[l] / 4;
             const int blocks = (int)std::min<size_t>(4096, (n4 + 255) / 256);
