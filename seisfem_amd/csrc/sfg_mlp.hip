@@ -76,7 +76,10 @@ __device__ __forceinline__ float adam_update(float w, float g, float &m, float &
 //   EPI_MSE        d = v + bias[col] - T[tidx[row]][col]; C = scale d and H = bf16(scale d) on
 //                  columns < nvalid (0 beyond); *loss += sum d^2 (the mean-squared-error
 //                  gradient of the output layer, targets read in place by sample index)
-enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_MSE = 3 };
+//   EPI_ADAM       v is the gradient of weight W[row][col] (row stride ldc, moments Mw, Vw
+//                  alike): one Adam step on rows < arows, columns < acols, and the bf16
+//                  copy W16 refreshed; C is not written (row-piece epilogue only)
+enum Epi { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_MSE = 3, EPI_ADAM = 4 };
 struct EpiArgs {
     int kind = EPI_STORE;
     const float *bias = nullptr;
@@ -88,6 +91,10 @@ struct EpiArgs {
     int nvalid = 0;
     float scale = 0.0f;
     double *loss = nullptr;
+    float *W = nullptr, *Mw = nullptr, *Vw = nullptr;
+    uint16_t *W16 = nullptr;
+    int arows = 0, acols = 0;
+    AdamArgs ad{};
 };
 
 // ---- C = A . B^T, bf16 in, fp32 out -----------------------------------------
@@ -300,8 +307,9 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
         const int col = n0 + wn * WC + pc * 4, r0 = m0 + wm * (BM / WM) + pr;
         const bool mse = ep.kind == EPI_MSE, act = ep.kind == EPI_BIAS_GELU;
         const int nv = mse ? ep.nvalid : N; // a multiple of 4: a piece is all valid or all not
+        const bool adam = ep.kind == EPI_ADAM, biased = ep.kind != EPI_STORE && !adam;
         float4 bc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (ep.kind != EPI_STORE) bc = *reinterpret_cast<const float4 *>(&ep.bias[min(col, N - 4)]);
+        if (biased) bc = *reinterpret_cast<const float4 *>(&ep.bias[min(col, N - 4)]);
         double lsum = 0.0;
 #pragma unroll
         for (int a = 0; a < TM; ++a) {
@@ -322,14 +330,41 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
                 for (int i = 0; i < PL; ++i)
                     tv[i] = *reinterpret_cast<const float4 *>(ep.T + (size_t)ti[i] * ep.ldt + min(col, nv - 4));
             }
+            // Adam: the pass's weights and moments, likewise all issued before any store
+            float4 aw[PL], am[PL], av[PL];
+            if (adam) {
+#pragma unroll
+                for (int i = 0; i < PL; ++i) {
+                    const size_t o = (size_t)min(r0 + a * 32 + i * RS, ep.arows - 1) * ldc + min(col, N - 4);
+                    aw[i] = *reinterpret_cast<const float4 *>(ep.W + o);
+                    am[i] = *reinterpret_cast<const float4 *>(ep.Mw + o);
+                    av[i] = *reinterpret_cast<const float4 *>(ep.Vw + o);
+                }
+            }
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < PL; ++i) {
                 const int row = r0 + a * 32 + i * RS;
                 const float4 v = *reinterpret_cast<const float4 *>(&S[(pr + i * RS) * SLD + pc * 4]);
                 if (row >= M || col >= N) continue;
+                if (adam) {
+                    if (row >= ep.arows) continue;
+                    float4 w4 = aw[i], m4 = am[i], v4 = av[i];
+                    float *pw = &w4.x, *pm = &m4.x, *pv = &v4.x;
+                    const float *pg = &v.x;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (col + e < ep.acols) pw[e] = adam_update(pw[e], pg[e], pm[e], pv[e], ep.ad);
+                    const size_t o = (size_t)row * ldc + col;
+                    *reinterpret_cast<float4 *>(ep.W + o) = w4;
+                    *reinterpret_cast<float4 *>(ep.Mw + o) = m4;
+                    *reinterpret_cast<float4 *>(ep.Vw + o) = v4;
+                    *reinterpret_cast<uint2 *>(ep.W16 + o) = make_uint2(f2bf(w4.x) | ((uint32_t)f2bf(w4.y) << 16),
+                                                                        f2bf(w4.z) | ((uint32_t)f2bf(w4.w) << 16));
+                    continue;
+                }
                 float4 o = v;
-                if (ep.kind != EPI_STORE) o = make_float4(v.x + bc.x, v.y + bc.y, v.z + bc.z, v.w + bc.w);
+                if (biased) o = make_float4(v.x + bc.x, v.y + bc.y, v.z + bc.z, v.w + bc.w);
                 if (mse) {
                     const float d[4] = {o.x - tv[i].x, o.y - tv[i].y, o.z - tv[i].z, o.w - tv[i].w};
                     float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -397,7 +432,7 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
         });
         for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
         if (lane == 0) atomicAdd(ep.loss, lsum);
-    } else {
+    } else if (ep.kind != EPI_ADAM) { // (the host sends EPI_ADAM only where the row pieces apply)
         const bool act = ep.kind == EPI_BIAS_GELU;
         tiles([&](const f32x16 &t, int col, int rb) __attribute__((always_inline)) {
             const float bc = col < N ? ep.bias[col] : 0.0f;
@@ -537,54 +572,6 @@ __global__ __launch_bounds__(256) void bias_grad_kernel(int batch, int cols, int
     }
 }
 
-// Adam on FP32 master weights [rows][ld] (valid columns < cols, ld a multiple
-// of 4), four weights per piece, two pieces per thread and trip; refreshes the
-// bf16 copy w16.  Every byte is touched once: non-temporal loads and stores,
-// and both pieces' loads (from clamped, in-bounds indices) are issued before
-// any store, so that a thread keeps two pieces' reads in flight.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-__global__ void adam_kernel(int rows, int cols, int ld, float *__restrict__ w, const float *__restrict__ g,
-                            float *__restrict__ m, float *__restrict__ v, AdamArgs a, uint16_t *__restrict__ w16)
-{
-    constexpr int U = 2;
-    const size_t total4 = (size_t)rows * ld / 4, stride = (size_t)gridDim.x * blockDim.x;
-    f32x4 *w4 = reinterpret_cast<f32x4 *>(w), *m4 = reinterpret_cast<f32x4 *>(m), *v4 = reinterpret_cast<f32x4 *>(v);
-    const f32x4 *g4 = reinterpret_cast<const f32x4 *>(g);
-    uint2 *h4 = reinterpret_cast<uint2 *>(w16);
-    for (size_t q0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q0 < total4; q0 += U * stride) {
-        f32x4 W[U], Mm[U], V[U], G[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const size_t q = min(q0 + u * stride, total4 - 1);
-            W[u] = __builtin_nontemporal_load(&w4[q]);
-            Mm[u] = __builtin_nontemporal_load(&m4[q]);
-            V[u] = __builtin_nontemporal_load(&v4[q]);
-            G[u] = __builtin_nontemporal_load(&g4[q]);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const size_t q = q0 + u * stride;
-            if (q >= total4) break;
-            const int c = (int)((q * 4) % ld);
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (c + e < cols) {
-                    float pm = Mm[u][e], pv = V[u][e];
-                    W[u][e] = adam_update(W[u][e], G[u][e], pm, pv, a);
-                    Mm[u][e] = pm;
-                    V[u][e] = pv;
-                }
-            __builtin_nontemporal_store(W[u], &w4[q]);
-            __builtin_nontemporal_store(Mm[u], &m4[q]);
-            __builtin_nontemporal_store(V[u], &v4[q]);
-            const uint2 h = make_uint2((uint32_t)f2bf(W[u][0]) | ((uint32_t)f2bf(W[u][1]) << 16),
-                                       (uint32_t)f2bf(W[u][2]) | ((uint32_t)f2bf(W[u][3]) << 16));
-            __builtin_nontemporal_store(h.x, &h4[q].x);
-            __builtin_nontemporal_store(h.y, &h4[q].y);
-        }
-    }
-}
-
 int pad32(int x) { return (x + 31) / 32 * 32; }
 
 thread_local std::string g_mlp_err;
@@ -641,6 +628,14 @@ hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int lda,
     if (K % 8 || lda % 8 || ldb % 8 || M <= 0 || N <= 0 || K <= 0) return hipErrorInvalidValue;
     if (tb && N % 8) return hipErrorInvalidValue;
     if (ep.kind == EPI_MSE && !ep.H) return hipErrorInvalidValue;
+    if (ep.kind == EPI_ADAM) {
+        // the row-piece epilogue must apply: 16-byte rows of C (unused), W and its moments
+        auto a16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+        if (N % 4 || ldc % 4 || !a16(C) || !ep.W || !a16(ep.W) || !ep.Mw || !a16(ep.Mw) || !ep.Vw || !a16(ep.Vw) ||
+            !ep.W16 || (reinterpret_cast<uintptr_t>(ep.W16) & 7) || ep.H || ep.arows < 1 || ep.arows > M ||
+            ep.acols < 1 || ep.acols > N)
+            return hipErrorInvalidValue;
+    }
     const GemmShape gs = gemm_shape(M, N, K, ep.kind);
     const bool wide = gs.wide, big = gs.big;
     const int bm = gs.bm, bn = gs.bn, splits = gs.splits, kchunk = gs.kchunk;
@@ -867,7 +862,7 @@ static int backward(sfg_mlp *m, const AdamArgs *ad = nullptr)
             zl.n[zl.count++] = (size_t)B * m->pin[l];
             zx[l] = 1;
         }
-        if (gemm_shape(m->pout[l], m->pin[l], B, EPI_STORE).splits > 1) {
+        if (!ad && gemm_shape(m->pout[l], m->pin[l], B, EPI_STORE).splits > 1) {
             zl.p[zl.count] = m->G[l];
             zl.n[zl.count++] = (size_t)m->pout[l] * m->pin[l];
             zw[l] = 1;
@@ -896,14 +891,21 @@ static int backward(sfg_mlp *m, const AdamArgs *ad = nullptr)
         // element by element costs as much as the products at this small K)
         MCHK(transpose2(m->st, Transpose{B, m->pout[l], m->dz16[l], m->pout[l], m->dzT16[l], B},
                         Transpose{B, m->pin[l], m->h16[l], m->pin[l], m->hT16[l], B}));
-        MCHK(gemm(m->st, m->pout[l], m->pin[l], B, m->dzT16[l], B, m->hT16[l], B, m->G[l], m->pin[l], EpiArgs{}, false,
-                  zw[l]));
         if (ad) {
-            const size_t n4 = (size_t)m->dout[l] * m->pin[l] / 4;
-            const int blocks = (int)std::min<size_t>(4096, (n4 + 255) / 256);
-            hipLaunchKernelGGL(adam_kernel, dim3(blocks), dim3(256), 0, m->st, m->dout[l], m->din[l], m->pin[l],
-                               m->W[l], m->G[l], m->Mw[l], m->Vw[l], *ad, m->W16[l]);
-            MCHK(hipGetLastError());
+            // the weights' Adam step in the dW product's epilogue: the gradient is never stored
+            EpiArgs ep;
+            ep.kind = EPI_ADAM;
+            ep.W = m->W[l];
+            ep.Mw = m->Mw[l];
+            ep.Vw = m->Vw[l];
+            ep.W16 = m->W16[l];
+            ep.arows = m->dout[l];
+            ep.acols = m->din[l];
+            ep.ad = *ad;
+            MCHK(gemm(m->st, m->pout[l], m->pin[l], B, m->dzT16[l], B, m->hT16[l], B, m->G[l], m->pin[l], ep));
+        } else {
+            MCHK(gemm(m->st, m->pout[l], m->pin[l], B, m->dzT16[l], B, m->hT16[l], B, m->G[l], m->pin[l], EpiArgs{},
+                      false, zw[l]));
         }
     }
     return SFG_OK;
