@@ -5,7 +5,7 @@
  * accumulation, GELU, MSE, Adam with FP32 master weights).
  *
  * No counterpart in the reference (it has no learned component); the FEM
- * loop generates the training shots and judges the surrogate's error.
+ * loop generates the training shots and measures the surrogate's error.
  * Host pointers throughout; returns SFG_OK or a negative SFG_E_* code
  * (message in sfg_mlp_last_error()).
  */

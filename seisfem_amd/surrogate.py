@@ -1,4 +1,4 @@
-"""Learned surrogate of the FEM loop, with the FEM as fallback and judge.
+"""Learned surrogate of the FEM loop, with the FEM as fallback and as the check on its error.
 
 A layered isotropic model (3 interface depths, 4 P velocities; vs = vp/1.73,
 Gardner density) plus a source position map, through an MLP trained on the
@@ -260,7 +260,7 @@ class FEMGenerator:
             self.op = None
 
 
-# ---- training, the error gate and the judged evaluation ---------------------------
+# ---- training, the error gate and the FEM-checked evaluation ----------------------
 
 def rel_l2_rows(A, B):
     return np.linalg.norm(A - B, axis=1) / np.maximum(np.linalg.norm(B, axis=1), 1e-30)
@@ -387,10 +387,10 @@ class ErrorGate:
 
 def evaluate_gated(nets, gate, Xtest, Ytest, fem_seconds_per_shot, audit_frac=0.1, min_audit=10, seed=0,
                    mirror_nrec=0):
-    """The surrogate with the FEM as fallback and judge, on held-out models.
+    """The surrogate with the FEM as fallback and as its check, on held-out models.
     Every model gets the ensemble's prediction; the gate accepts it or sends the
     model to the FEM, and the FEM also re-runs a random sample of the accepted
-    models (audit_frac of them, at least min_audit) to judge the gate.  The
+    models (audit_frac of them, at least min_audit) to check the gate.  The
     held-out FEM traces are known here, so the report also gives the gate's
     true precision over all accepted models.  Combined rate = models / (device
     time of the predictions + FEM time of the rejected and audited models).
