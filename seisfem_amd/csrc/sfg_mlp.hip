@@ -95,6 +95,9 @@ struct EpiArgs {
     uint16_t *W16 = nullptr;
     int arows = 0, acols = 0;
     AdamArgs ad{};
+    // a product split over K (gridDim.z > 1): 0 = the parts are added into C with
+    // atomics; > 0 = part z is stored (EPI_STORE) at C + z * slice
+    size_t slice = 0;
 };
 
 // ---- C = A . B^T, bf16 in, fp32 out -----------------------------------------
@@ -280,7 +283,8 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
     };
     // row of register r in a tile whose lane rows start at rb
 #define SFG_ROW(r) (rb + ((r) >> 2) * 8 + ((r) & 3))
-    if (gridDim.z > 1) {
+    if (gridDim.z > 1) C += (size_t)blockIdx.z * ep.slice;
+    if (gridDim.z > 1 && ep.slice == 0) {
         tiles([&](const f32x16 &t, int col, int rb) __attribute__((always_inline)) {
 #pragma unroll
             for (int r = 0; r < 16; ++r)
@@ -533,13 +537,33 @@ __global__ void gather_rows_kernel(int batch, int width, int ld, const float *sr
     }
 }
 
-// dz = dh * gelu'(z) (fp32 and bf16 copies)
-__global__ void gelu_back_kernel(size_t n, const float *dh, const float *z, float *dz, uint16_t *dz16)
+// dz = dh * gelu'(z) (fp32 and bf16 copies), dh the sum of the nparts parts of
+// a product split over K (part p at parts + p * slice), added in part order
+__global__ void gelu_back_kernel(size_t n, const float *parts, int nparts, size_t slice, const float *z, float *dz,
+                                 uint16_t *dz16)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const float g = dh[i] * gelu_grad(z[i]);
+        float dh = parts[i];
+        for (int p = 1; p < nparts; ++p) dh += parts[p * slice + i];
+        const float g = dh * gelu_grad(z[i]);
         dz[i] = g;
         dz16[i] = f2bf(g);
+    }
+}
+
+// A hidden layer's forward epilogue after a product split over K: z = (sum of the
+// nparts parts, in part order) + bias[c], h = bf16(gelu(z)); rows x cols, row
+// stride cols for all of them
+__global__ void bias_gelu_parts_kernel(int rows, int cols, const float *parts, int nparts, size_t slice,
+                                       const float *bias, float *z, uint16_t *h)
+{
+    const int n = rows * cols;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float v = parts[i];
+        for (int p = 1; p < nparts; ++p) v += parts[p * slice + i];
+        const float zz = v + bias[i % cols];
+        z[i] = zz;
+        h[i] = f2bf(gelu(zz));
     }
 }
 
@@ -639,7 +663,8 @@ hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int lda,
     const GemmShape gs = gemm_shape(M, N, K, ep.kind);
     const bool wide = gs.wide, big = gs.big;
     const int bm = gs.bm, bn = gs.bn, splits = gs.splits, kchunk = gs.kchunk;
-    if (splits > 1 && !zeroed)
+    if (ep.slice && (ep.kind != EPI_STORE || ep.slice < (size_t)M * ldc || ep.slice % 4)) return hipErrorInvalidValue;
+    if (splits > 1 && !zeroed && !ep.slice)
         hipLaunchKernelGGL(fill_zero_f32_kernel, dim3(1024), dim3(256), 0, st, M, N, ldc, C);
     const dim3 g((N + bn - 1) / bn, (M + bm - 1) / bm, splits);
 #define SFG_GEMM(BMV, TBV)                                                                                        \
@@ -684,6 +709,8 @@ struct sfg_mlp {
     std::vector<uint16_t *> h16, hT16;
     std::vector<float *> z, dz, dh;
     std::vector<uint16_t *> dz16, dzT16;
+    float *part = nullptr; // parts of the products split over K (dX; hidden layers' forward)
+    size_t part_n = 0;
     float *y = nullptr;
     double *loss = nullptr;
     int *idx = nullptr;   // sample indices of one batch
@@ -703,7 +730,8 @@ static void mlp_free(sfg_mlp *m)
     };
     F(m->W); F(m->G); F(m->Mw); F(m->Vw); F(m->b); F(m->gb); F(m->Mb); F(m->Vb);
     F(m->W16); F(m->h16); F(m->hT16); F(m->z); F(m->dz); F(m->dh); F(m->dz16); F(m->dzT16);
-    for (void *p : {(void *)m->y, (void *)m->loss, (void *)m->idx, (void *)m->order, (void *)m->Xd, (void *)m->Yd})
+    for (void *p : {(void *)m->y, (void *)m->part, (void *)m->loss, (void *)m->idx, (void *)m->order, (void *)m->Xd,
+                    (void *)m->Yd})
         if (p) (void)hipFree(p);
     if (m->st) (void)hipStreamDestroy(m->st);
     delete m;
@@ -779,6 +807,16 @@ extern "C" int sfg_mlp_create(sfg_mlp **out, int nin, int hidden, int nhidden_la
         m->dh.push_back(dhh); m->dz16.push_back(d16); m->dzT16.push_back(dT);
     }
     if (e == hipSuccess) e = zalloc(&m->y, (size_t)batch * m->pout.back());
+    // room for the parts of the largest product split over K
+    for (int l = 0; l < m->nl; ++l) {
+        if (l > 0)
+            m->part_n = std::max(m->part_n, (size_t)gemm_shape(batch, m->pin[l], m->pout[l], EPI_STORE).splits *
+                                                batch * m->pin[l]);
+        if (l < m->nl - 1)
+            m->part_n = std::max(m->part_n, (size_t)gemm_shape(batch, m->pout[l], m->pin[l], EPI_STORE).splits *
+                                                batch * m->pout[l]);
+    }
+    if (e == hipSuccess) e = zalloc(&m->part, m->part_n);
     if (e == hipSuccess) e = zalloc(&m->loss, 1);
     if (e == hipSuccess) e = zalloc(&m->idx, batch);
     // bf16 weights from the master copy
@@ -817,6 +855,20 @@ static int forward(sfg_mlp *m, const float *Y = nullptr, const int *idx = nullpt
     if (Y) MCHK(hipMemsetAsync(m->loss, 0, sizeof(double), m->st));
     for (int l = 0; l < m->nl; ++l) {
         const bool last = l == m->nl - 1;
+        if (!last) {
+            const GemmShape gs = gemm_shape(m->batch, m->pout[l], m->pin[l], EPI_STORE);
+            if (gs.splits > 1) {
+                // few tiles and a long K: the product in parts over K, then bias + GELU
+                EpiArgs ep;
+                ep.slice = (size_t)m->batch * m->pout[l];
+                MCHK(gemm(m->st, m->batch, m->pout[l], m->pin[l], m->h16[l], m->pin[l], m->W16[l], m->pin[l], m->part,
+                          m->pout[l], ep));
+                hipLaunchKernelGGL(bias_gelu_parts_kernel, dim3(1024), dim3(256), 0, m->st, m->batch, m->pout[l],
+                                   m->part, gs.splits, ep.slice, m->b[l], m->z[l], m->h16[l + 1]);
+                MCHK(hipGetLastError());
+                continue;
+            }
+        }
         EpiArgs ep;
         ep.bias = m->b[l];
         float *C = m->z[l];
@@ -852,16 +904,12 @@ static int forward(sfg_mlp *m, const float *Y = nullptr, const int *idx = nullpt
 static int backward(sfg_mlp *m, const AdamArgs *ad = nullptr)
 {
     const int B = m->batch;
-    // the products split over K (dX of most layers, dW of narrow ones) accumulate
-    // into their outputs: all of those are zeroed by one launch up front
+    // without an Adam step, the dW products split over K (narrow layers) add into
+    // their outputs: all of those are zeroed by one launch up front (the dX
+    // products split over K store their parts, which gelu_back adds up)
     ZeroList zl{};
-    std::vector<char> zx(m->nl, 0), zw(m->nl, 0);
-    for (int l = m->nl - 1; l >= 0 && zl.count < 7; --l) {
-        if (l > 0 && gemm_shape(B, m->pin[l], m->pout[l], EPI_STORE).splits > 1) {
-            zl.p[zl.count] = m->dh[l];
-            zl.n[zl.count++] = (size_t)B * m->pin[l];
-            zx[l] = 1;
-        }
+    std::vector<char> zw(m->nl, 0);
+    for (int l = m->nl - 1; l >= 0 && zl.count < 8; --l) {
         if (!ad && gemm_shape(m->pout[l], m->pin[l], B, EPI_STORE).splits > 1) {
             zl.p[zl.count] = m->G[l];
             zl.n[zl.count++] = (size_t)m->pout[l] * m->pin[l];
@@ -874,12 +922,20 @@ static int backward(sfg_mlp *m, const AdamArgs *ad = nullptr)
     }
     for (int l = m->nl - 1; l >= 0; --l) {
         if (l > 0) {
-            // dh = dz . W (W [pout][pin] read k-major as stored) ; dz_{l-1} = dh * gelu'(z_{l-1})
-            MCHK(gemm(m->st, B, m->pin[l], m->pout[l], m->dz16[l], m->pout[l], m->W16[l], m->pin[l], m->dh[l],
-                      m->pin[l], EpiArgs{}, true, zx[l]));
+            // dh = dz . W (W [pout][pin] read k-major as stored) ; dz_{l-1} = dh * gelu'(z_{l-1});
+            // split over K, the product's parts go to m->part and gelu_back adds them up
+            const GemmShape gs = gemm_shape(B, m->pin[l], m->pout[l], EPI_STORE);
             const size_t n = (size_t)B * m->pin[l];
-            hipLaunchKernelGGL(gelu_back_kernel, dim3(1024), dim3(256), 0, m->st, n, m->dh[l], m->z[l - 1],
-                               m->dz[l - 1], m->dz16[l - 1]);
+            EpiArgs ep;
+            float *dh = m->dh[l];
+            if (gs.splits > 1) {
+                ep.slice = n;
+                dh = m->part;
+            }
+            MCHK(gemm(m->st, B, m->pin[l], m->pout[l], m->dz16[l], m->pout[l], m->W16[l], m->pin[l], dh, m->pin[l], ep,
+                      true));
+            hipLaunchKernelGGL(gelu_back_kernel, dim3(1024), dim3(256), 0, m->st, n, dh, gs.splits, ep.slice,
+                               m->z[l - 1], m->dz[l - 1], m->dz16[l - 1]);
             MCHK(hipGetLastError());
         }
         hipLaunchKernelGGL(bias_grad_kernel, dim3((m->pout[l] + 63) / 64), dim3(256), 0, m->st, B, m->dout[l],
