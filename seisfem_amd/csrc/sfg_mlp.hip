@@ -106,9 +106,9 @@ struct EpiArgs {
 // tiles are double-buffered: while a wave multiplies step k out of one buffer,
 // its share of step k+1 is loaded from global memory into registers, then
 // stored into the other buffer; one barrier per step.  blockIdx.z splits K
-// into chunks of kchunk (a multiple of BK): with more than one chunk the
-// partial products are added to C (zeroed beforehand) with atomics and no
-// epilogue other than the store applies.
+// into chunks of kchunk (a multiple of BK): with more than one chunk, chunk z's
+// partial product is stored at C + z * ep.slice (ep.slice > 0), or added to C
+// (zeroed beforehand) with atomics; no epilogue other than the store applies.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int BK = 64, LDK = BK + 8;
 
@@ -297,10 +297,10 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
                 (ep.ldt % 4 == 0 && ep.nvalid % 4 == 0 && (reinterpret_cast<uintptr_t>(ep.T) & 15) == 0))) {
         // Each wave turns its tile rows round in LDS (the A buffers, idle after the
         // last step's barrier), 32 rows at a time, so that the epilogue reads and
-        // writes whole rows in 4-element pieces (16 bytes of C, bias and targets,
-        // 8 bytes of H) rather than single elements: a wave's 32 x WC slab is
-        // written to LDS element by element in the register layout, then read back
-        // along the rows.  The arithmetic per element is that of the paths below.
+        // writes whole rows in 4-element pieces (16 bytes of C, bias, targets,
+        // weights and moments, 8 bytes of bf16) rather than single elements: a
+        // wave's 32 x WC slab is written to LDS element by element in the register
+        // layout, then read back along the rows.  The arithmetic per element is that of the paths below.
         constexpr int WC = BN / WN, SLD = WC + (WC >= 64 ? 8 : 4); // floats per staged row
         constexpr int PR = WC / 4, PL = 32 * PR / 64, RS = 64 / PR;   // pieces per row and per lane, rows apart
         static_assert(NW * 32 * SLD * 4 <= (int)sizeof(As), "staging slab fits in the A buffers");
