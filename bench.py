@@ -167,7 +167,10 @@ def mlp_leg():
     macs = sum(a * b for a, b in zip(widths[:-1], widths[1:]))
     # forward 2 flops per MAC; backward: dW for every layer and dX for all but the first
     train_flops = 2.0 * B * (3 * macs - widths[0] * widths[1])
-    g_ms = surrogate.gemm_bf16_time(B, nout, hid, reps=20)
+    # the GEMM alone, back to back: 100 untimed launches (the clock settles over the
+    # first few hundred), then the average of 500
+    surrogate.gemm_bf16_time(B, nout, hid, reps=100)
+    g_ms = surrogate.gemm_bf16_time(B, nout, hid, reps=500)
     g_tf = 2.0 * B * nout * hid / (g_ms * 1e-3) / 1e12
     out = {"network": f"{nin}-{hid}x{nh}-{nout}, batch {B}, synthetic data", "params": net_params(widths),
            "train_ms_per_step": ms, "train_TFLOPs": train_flops / (ms * 1e-3) / 1e12,
