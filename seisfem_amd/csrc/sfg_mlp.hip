@@ -98,6 +98,10 @@ struct EpiArgs {
     // a product split over K (gridDim.z > 1): 0 = the parts are added into C with
     // atomics; > 0 = part z is stored (EPI_STORE) at C + z * slice
     size_t slice = 0;
+    // EPI_MSE with db: the block holds every row (M <= BM), so it also sums the
+    // gradient over the rows into db[col] (col < nvalid) instead of storing C, and
+    // with bw updates the bias bw by Adam (moments bm, bv; arguments ad) from it
+    float *db = nullptr, *bw = nullptr, *bm = nullptr, *bv = nullptr;
 };
 
 // ---- C = A . B^T, bf16 in, fp32 out -----------------------------------------
@@ -315,6 +319,7 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
         float4 bc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (biased) bc = *reinterpret_cast<const float4 *>(&ep.bias[min(col, N - 4)]);
         double lsum = 0.0;
+        float cs[4] = {0.0f, 0.0f, 0.0f, 0.0f}; // the lane's column sums of the loss gradient (ep.db)
 #pragma unroll
         for (int a = 0; a < TM; ++a) {
 #pragma unroll
@@ -377,11 +382,12 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
                         for (int e = 0; e < 4; ++e) {
                             lsum += (double)d[e] * d[e];
                             g[e] = ep.scale * d[e];
+                            cs[e] += g[e];
                         }
                     }
                     o = make_float4(g[0], g[1], g[2], g[3]);
                 }
-                *reinterpret_cast<float4 *>(&C[(size_t)row * ldc + col]) = o;
+                if (!ep.db) *reinterpret_cast<float4 *>(&C[(size_t)row * ldc + col]) = o;
                 if (ep.H) {
                     const float h[4] = {act ? gelu(o.x) : o.x, act ? gelu(o.y) : o.y, act ? gelu(o.z) : o.z,
                                         act ? gelu(o.w) : o.w};
@@ -392,15 +398,38 @@ __global__ __launch_bounds__(NW * 64) void gemm_nt_kernel(int M, int N, int K, i
             __syncthreads();
         }
         if (mse) {
-            // one atomic per block: the waves' sums meet in LDS (free after the last pass's barrier)
+            // one atomic per block: the waves' sums meet in LDS (free after the last
+            // pass's barrier); so do the column sums of the gradient (ep.db): first
+            // over the lanes that hold the same columns, then over the WM waves
+            static_assert(NW * 8 + WM * BN * 4 + 128 <= (int)sizeof(Bs), "loss and column sums fit");
             for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
             double *ws = reinterpret_cast<double *>(&Bs[0][0]);
+            float *red = reinterpret_cast<float *>(&Bs[0][0]) + 32;
             if (lane == 0) ws[wave] = lsum;
+            if (ep.db) {
+#pragma unroll
+                for (int o = PR; o < 64; o <<= 1)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) cs[e] += __shfl_xor(cs[e], o, 64);
+                if (pr == 0)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) red[wm * BN + wn * WC + pc * 4 + e] = cs[e];
+            }
             __syncthreads();
             if (tid == 0) {
                 double t = 0.0;
                 for (int w = 0; w < NW; ++w) t += ws[w];
                 atomicAdd(ep.loss, t);
+            }
+            if (ep.db) {
+                for (int c = tid; c < BN; c += NT) {
+                    const int gc = n0 + c;
+                    if (gc >= ep.nvalid) continue;
+                    float g = red[c];
+                    for (int w = 1; w < WM; ++w) g += red[w * BN + c];
+                    ep.db[gc] = g;
+                    if (ep.bw) ep.bw[gc] = adam_update(ep.bw[gc], g, ep.bm[gc], ep.bv[gc], ep.ad);
+                }
             }
         }
     } else if (ep.kind == EPI_STORE) {
@@ -662,6 +691,14 @@ hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int lda,
             return hipErrorInvalidValue;
     }
     const GemmShape gs = gemm_shape(M, N, K, ep.kind);
+    if (ep.db) {
+        // the row-piece loss epilogue must apply, in blocks that hold every row
+        auto al = [](const void *p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0; };
+        if (ep.kind != EPI_MSE || M > gs.bm || N % 4 || ldc % 4 || !al(C, 16) || !ep.bias || !al(ep.bias, 16) ||
+            ep.ldt % 4 || ep.nvalid % 4 || ep.nvalid > N || !ep.T || !al(ep.T, 16) || ep.ldh % 4 || !al(ep.H, 8) ||
+            (ep.bw && (!ep.bm || !ep.bv)))
+            return hipErrorInvalidValue;
+    }
     const bool wide = gs.wide, big = gs.big;
     const int bm = gs.bm, bn = gs.bn, splits = gs.splits, kchunk = gs.kchunk;
     if (ep.slice && (ep.kind != EPI_STORE || ep.slice < (size_t)M * ldc || ep.slice % 4)) return hipErrorInvalidValue;
@@ -712,6 +749,7 @@ struct sfg_mlp {
     std::vector<uint16_t *> dz16, dzT16;
     float *part = nullptr; // parts of the products split over K (dX; hidden layers' forward)
     size_t part_n = 0;
+    bool out_db = false;   // the last forward formed the output layer's bias gradient
     float *y = nullptr;
     double *loss = nullptr;
     int *idx = nullptr;   // sample indices of one batch
@@ -851,8 +889,9 @@ extern "C" long long sfg_mlp_param_count(const sfg_mlp *m)
 // output goes to y; with targets Y (rows by the batch's sample indices idx, on
 // the device) the output layer's epilogue turns it into the loss gradient dz /
 // dz16 of the last layer and adds the squared errors to *loss (zeroed here).
-static int forward(sfg_mlp *m, const float *Y = nullptr, const int *idx = nullptr)
+static int forward(sfg_mlp *m, const float *Y = nullptr, const int *idx = nullptr, const AdamArgs *ad = nullptr)
 {
+    m->out_db = false;
     if (Y) MCHK(hipMemsetAsync(m->loss, 0, sizeof(double), m->st));
     for (int l = 0; l < m->nl; ++l) {
         const bool last = l == m->nl - 1;
@@ -891,6 +930,18 @@ static int forward(sfg_mlp *m, const float *Y = nullptr, const int *idx = nullpt
             ep.nvalid = m->nout;
             ep.scale = 2.0f / ((float)m->batch * (float)m->nout);
             ep.loss = m->loss;
+            // one block holds the whole batch: the bias gradient (and with ad the
+            // bias's Adam step) in the same epilogue, the FP32 gradient not stored
+            if (m->batch <= gemm_shape(m->batch, m->pout[l], m->pin[l], EPI_MSE).bm && m->nout % 4 == 0) {
+                ep.db = m->gb[l];
+                if (ad) {
+                    ep.bw = m->b[l];
+                    ep.bm = m->Mb[l];
+                    ep.bv = m->Vb[l];
+                    ep.ad = *ad;
+                }
+                m->out_db = true;
+            }
         }
         MCHK(gemm(m->st, m->batch, m->pout[l], m->pin[l], m->h16[l], m->pin[l], m->W16[l], m->pin[l], C, m->pout[l],
                   ep));
@@ -939,10 +990,12 @@ static int backward(sfg_mlp *m, const AdamArgs *ad = nullptr)
                                m->z[l - 1], m->dz[l - 1], m->dz16[l - 1]);
             MCHK(hipGetLastError());
         }
-        hipLaunchKernelGGL(bias_grad_kernel, dim3((m->pout[l] + 63) / 64), dim3(256), 0, m->st, B, m->dout[l],
-                           m->pout[l], m->dz[l], m->gb[l], ad ? m->b[l] : nullptr, m->Mb[l], m->Vb[l],
-                           ad ? *ad : AdamArgs{});
-        MCHK(hipGetLastError());
+        if (!(l == m->nl - 1 && m->out_db)) { // (else formed in the forward pass's loss epilogue)
+            hipLaunchKernelGGL(bias_grad_kernel, dim3((m->pout[l] + 63) / 64), dim3(256), 0, m->st, B, m->dout[l],
+                               m->pout[l], m->dz[l], m->gb[l], ad ? m->b[l] : nullptr, m->Mb[l], m->Vb[l],
+                               ad ? *ad : AdamArgs{});
+            MCHK(hipGetLastError());
+        }
         // dW = dz^T . h from transposed copies (A = dz^T [pout][B], B = h^T [pin][B]); K = batch.
         // Reading both operands k-major instead measured 2.4x slower (transposing into LDS
         // element by element costs as much as the products at this small K)
@@ -1030,11 +1083,10 @@ extern "C" int sfg_mlp_train_step(sfg_mlp *m, const int *idx, float lr, double *
     if (!m || !idx || !m->Xd) return mfail(SFG_E_ARG, "mlp train: no data");
     int rc = upload_idx(m, idx, m->ndata);
     if (!rc) rc = load_batch(m, m->idx, m->Xd);
-    if (!rc) rc = forward(m, m->Yd, m->idx);
-    if (!rc) {
-        const AdamArgs ad = next_adam(m, lr);
-        rc = backward(m, &ad);
-    }
+    AdamArgs ad{};
+    if (!rc) ad = next_adam(m, lr);
+    if (!rc) rc = forward(m, m->Yd, m->idx, &ad);
+    if (!rc) rc = backward(m, &ad);
     double h = 0.0;
     if (!rc) rc = read_loss(m, &h); // also ends the host index buffer's use
     if (!rc && loss) *loss = h;
@@ -1074,11 +1126,10 @@ extern "C" int sfg_mlp_train(sfg_mlp *m, int epochs, float lr0, long long total_
             const float lr = (float)(lr0 * 0.5 * (1.0 + std::cos(M_PI * prog)));
             const int *idx = m->order + (size_t)k * B;
             int rc = load_batch(m, idx, m->Xd);
-            if (!rc) rc = forward(m, m->Yd, idx);
-            if (!rc) {
-                const AdamArgs ad = next_adam(m, lr);
-                rc = backward(m, &ad);
-            }
+            AdamArgs ad{};
+            if (!rc) ad = next_adam(m, lr);
+            if (!rc) rc = forward(m, m->Yd, idx, &ad);
+            if (!rc) rc = backward(m, &ad);
             if (rc) return rc;
             ++done;
         }
