@@ -553,10 +553,12 @@ __global__ void f32_to_bf16_kernel(size_t n, const float *src, uint16_t *dst)
 }
 
 // rows of X (nin wide) for the given sample indices -> bf16 [batch][ldx], zero padded
+// (and *zero = 0, the batch's loss accumulator, by the first thread)
 __global__ void gather_rows_kernel(int batch, int width, int ld, const float *src, const int *idx, uint16_t *dst,
-                                   float *dstf)
+                                   float *dstf, double *zero)
 {
     const int b = blockIdx.y;
+    if (zero && b == 0 && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0.0;
     if (b >= batch) return;
     const size_t s = (size_t)idx[b];
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ld; c += gridDim.x * blockDim.x) {
@@ -885,14 +887,15 @@ extern "C" long long sfg_mlp_param_count(const sfg_mlp *m)
     return c;
 }
 
-// forward of the current batch (h16[0] holds its input).  Without targets the
-// output goes to y; with targets Y (rows by the batch's sample indices idx, on
-// the device) the output layer's epilogue turns it into the loss gradient dz /
-// dz16 of the last layer and adds the squared errors to *loss (zeroed here).
+// forward of the current batch (h16[0] holds its input, loaded by load_batch,
+// which also zeroes *loss).  Without targets the output goes to y; with targets
+// Y (rows by the batch's sample indices idx, on the device) the output layer's
+// epilogue turns it into the loss gradient dz16 (and dz) of the last layer, adds
+// the squared errors to *loss, and where it can forms the output bias's gradient
+// (and with ad takes its Adam step).
 static int forward(sfg_mlp *m, const float *Y = nullptr, const int *idx = nullptr, const AdamArgs *ad = nullptr)
 {
     m->out_db = false;
-    if (Y) MCHK(hipMemsetAsync(m->loss, 0, sizeof(double), m->st));
     for (int l = 0; l < m->nl; ++l) {
         const bool last = l == m->nl - 1;
         if (!last) {
@@ -1035,7 +1038,7 @@ static int upload_idx(sfg_mlp *m, const int *idx_host, int nsrc)
 static int load_batch(sfg_mlp *m, const int *idx, const float *X)
 {
     hipLaunchKernelGGL(gather_rows_kernel, dim3(1, m->batch), dim3(64), 0, m->st, m->batch, m->nin, m->pin[0], X,
-                       idx, m->h16[0], (float *)nullptr);
+                       idx, m->h16[0], (float *)nullptr, m->loss);
     MCHK(hipGetLastError());
     return SFG_OK;
 }
